@@ -1,0 +1,148 @@
+/*
+ * usg.h -- C ABI of the MI355X (gfx950) UnstructSolver residual-and-update
+ * path.  Plain pointers and sizes only; every call returns USG_OK (0) or a
+ * negative status and never throws.  Host arrays use the reference's
+ * array-of-structures layout (value of variable j of cell k at [k*nvar + j]),
+ * so a TopologyHolderStrategy adapter can hand its getU()/getResidual()
+ * buffers straight through; device storage is structure-of-arrays.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference checkout, SenhuWong/tmp).
+ */
+#ifndef USG_H
+#define USG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define USG_OK 0
+#define USG_ERR_ARG -1      /* bad argument (null pointer, size, enum)          */
+#define USG_ERR_MESH -2     /* invalid mesh (left cell < 0, unknown BC tag, ...) */
+#define USG_ERR_HIP -3      /* HIP runtime error                                */
+#define USG_ERR_NUMERIC -4  /* invariant violated on the device (HLLC p*, slip wall) */
+#define USG_ERR_NODEVICE -5 /* no GPU visible                                   */
+
+/* boundary codes of a face's right cell: toolBox/edge3d_int.h:33-40 */
+#define USG_WALL -2
+#define USG_FARFIELD -3
+
+/* flow type: WrapSolver::setParams "Euler2D" / "ViscousFlow2D" (SolverWrapper.h:30-41);
+ * USG_SST is ViscousFlow2D with isLaminar() == false (ViscousFlow2D.h:156-159). */
+#define USG_EULER 0
+#define USG_LAMINAR 1
+#define USG_SST 2
+
+/* precision of the residual kernels */
+#define USG_FP64 0  /* every kernel in FP64 (the drop-in)                      */
+#define USG_MIXED 1 /* gradients, limiter and fluxes in FP32; residual sums,
+                       LU-SGS diagonal inverse and norms in FP64               */
+
+/* fields readable with usg_get_field (host layout AoS, see each comment) */
+#define USG_F_PRIM 0     /* U   [nc*4]  (rho, p, u, v)     Euler2D::getU          */
+#define USG_F_CONS 1     /* W   [nc*4]  (rho, rhoE, rhou, rhov)  LUSGS W_scratch */
+#define USG_F_RES 2      /* R   [nc*4]                      getResidual           */
+#define USG_F_GRAD 3     /* gradU [nc*4*2]                  getGradientPrimitive  */
+#define USG_F_LAMC 4     /* spectral radius [nc]            getSpectrumConvective */
+#define USG_F_PHI 5      /* limiter [nc*4]                  Vankatakrishnan getLimiter */
+#define USG_F_DW1 6      /* forward-sweep increment [nc*4]  LUSGS d_deltaW1       */
+#define USG_F_DW 7       /* backward-sweep increment [nc*4] LUSGS d_deltaW        */
+#define USG_F_LAMV 8     /* viscous spectral radius [nc]    getSpectrumViscous    */
+#define USG_F_GRADT 9    /* temperature gradient [nc*2]     getGradientT          */
+#define USG_F_TURB 10    /* (k, omega) [nc*2]               SSTkomegaModel::getU  */
+#define USG_F_TRES 11    /* turbulence residual [nc*2]      SSTkomegaModel::getResidual */
+#define USG_F_MUT 12     /* eddy viscosity [nc]             SSTkomegaModel vCellMut */
+#define USG_F_F1 13      /* SST blending F1 [nc]                                  */
+#define USG_F_DT 14      /* local time step [nc]            getStableTime         */
+#define USG_F_TDW 15     /* turbulence backward increment [nc*2]                   */
+
+typedef struct usg_mesh_desc {
+  int nn;              /* nodes                                              */
+  int nf;              /* faces                                              */
+  int nc;              /* cells                                              */
+  const double *xy;    /* [nn*2] node coordinates                            */
+  const int *fnode;    /* [nf*2] 0-based face nodes; normal (dy,-dx) out of left cell */
+  const int *flc;      /* [nf] left cell (0-based)                           */
+  const int *frc;      /* [nf] right cell, or USG_WALL / USG_FARFIELD        */
+} usg_mesh_desc;
+
+typedef struct usg_params {
+  int flow;            /* USG_EULER / USG_LAMINAR / USG_SST                  */
+  int precision;       /* USG_FP64 / USG_MIXED                               */
+  double mach;         /* free-stream Mach number                            */
+  double aoa_deg;      /* angle of attack in degrees                         */
+  double reynolds;     /* Reynolds number (viscous flows)                    */
+  double cfl_lusgs;    /* 400 in the reference (LUSGS_Integrator.h:18)       */
+  double cfl_rk;       /* 0.8 in the reference (RungeKuttaStrategy.h:125)    */
+} usg_params;
+
+typedef struct usg_ctx usg_ctx;
+
+/* Context = one UnstructTopologyHolder + flow strategy + integrator on one
+ * HIP stream.  Replaces WrapSolver::setTopology/setParams/init
+ * (SolverWrapper.h:22-68): builds geometry (UnstructBlock.h:572-586), the
+ * free stream (TopologyHolderStrategy.cpp:325-351) and the LU-SGS wave
+ * schedule, uploads SoA arrays.  Contexts are independent. */
+int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out);
+int usg_ctx_destroy(usg_ctx *ctx);
+/* hipStream_t to launch on (NULL = the context's own stream). */
+int usg_set_stream(usg_ctx *ctx, void *hip_stream);
+int usg_sync(usg_ctx *ctx);
+
+/* Cold start: Euler2D::initializeData (Euler2D.cpp:153-171) and
+ * LUSGSIntegrator::initializeData (LUSGS_Integrator.cpp:107-111); SST adds
+ * SSTkomegaModel::InitializeturbulenceVariable (SSTKWStrategy.h:582-613). */
+int usg_init_freestream(usg_ctx *ctx);
+/* Warm start from primitive U [nc*4] (and, for SST, (k,omega) [nc*2], may be NULL). */
+int usg_set_prim(usg_ctx *ctx, const double *U, const double *turb);
+
+/* preprocessAdvanceSerial (Euler2D.h:133-140; ViscousFlow2D.h:192-212):
+ * boundary states, gradients, spectral radii, limiter, HLLC (+ viscous,
+ * + SST) residual of the current state. */
+int usg_residual(usg_ctx *ctx);
+/* n LU-SGS pseudo-time iterations: LUSGSIntegrator::singleStepSerial /
+ * UpdateSerial (LUSGS_Integrator.cpp:124-129,228-255).  res_hist (may be
+ * NULL) receives ||R_rho/V||_2 of each iteration's residual. */
+int usg_lusgs_step(usg_ctx *ctx, int n, double *res_hist);
+/* n RK5 steps: RungeKuttaStrategy::singleStepSerial (RungeKuttaStrategy.h:28-72). */
+int usg_rk5_step(usg_ctx *ctx, int n, double *res_hist);
+
+/* Copy a field to the host (AoS layout of the reference); count = elements. */
+int usg_get_field(usg_ctx *ctx, int field, double *out, long count);
+/* ||R_rho / V||_2 of the last residual (the build's convergence norm). */
+int usg_res_l2(usg_ctx *ctx, double *out);
+/* Lift and drag coefficients from the wall faces of the last residual. */
+int usg_forces(usg_ctx *ctx, double *cl, double *cd);
+
+/* Mesh facts: number of LU-SGS waves (forward, backward) of the numbering. */
+int usg_sweep_waves(usg_ctx *ctx, int *nfwd, int *nbwd);
+/* Host geometry as the reference computes it (cell3d/edge3d/UnstructBlock):
+ * vol[nc], cc[nc*2], fn[nf*2], fa[nf], fc[nf*2], wdist[nc]; any may be NULL. */
+int usg_get_geometry(usg_ctx *ctx, double *vol, double *cc, double *fn, double *fa,
+                     double *fc, double *wdist);
+
+/* Host-only mesh preparation (no GPU needed): the same geometry and LU-SGS
+ * wave schedule usg_ctx_create builds, for inspection and tests. */
+int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *fn, double *fa,
+                   double *fc, double *wdist, int *nfwd, int *nbwd);
+
+/* Kernel timing: when enabled, every launch of the dominant kernel (the cell
+ * flux/residual kernel) is bracketed by HIP events on its stream; the total
+ * and the count are read (and reset) with usg_kernel_time. */
+int usg_timing_enable(usg_ctx *ctx, int on);
+int usg_kernel_time(usg_ctx *ctx, double *total_ms, long *launches);
+
+/* Overset donor-stencil apply (UniTioga/dataUpdate.cpp:6-133,
+ * MeshBlock::getInterpolatedSolution + updateSolnData, ROW layout):
+ * receptor r takes q[recv[r]*nvar + k] = sum_{m<nw[r]} w[r*4+m] * q[donor[r*4+m]*nvar + k].
+ * Works on a caller-provided device array q of nq*nvar doubles. */
+int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const int *nw,
+                      const int *donor, const double *w, void *hip_stream);
+
+/* Last error text of the calling thread. */
+const char *usg_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

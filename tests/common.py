@@ -1,0 +1,59 @@
+"""Shared fixtures of the test suite: meshes, perturbed states, comparisons."""
+from __future__ import annotations
+
+import functools
+import os
+
+import numpy as np
+
+from tmp_amd import mesh
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REFERENCE = "/root/reference"
+
+
+@functools.lru_cache(maxsize=None)
+def part2():
+    """Shipped overset component fixture (2222 quads): wall tag -2, overset
+    tag -2147483647 treated as far field (Orchestra overTag)."""
+    return mesh.read_grd(os.path.join(GOLDEN, "part2.grd"), -2, -2147483647)
+
+
+@functools.lru_cache(maxsize=None)
+def bump():
+    """Channel with a 10 % circular-arc bump, ~20k mixed cells, seed 1001,
+    colour-blocked Hilbert numbering."""
+    return mesh.prepare(mesh.channel_bump(seed=1001))[0]
+
+
+@functools.lru_cache(maxsize=None)
+def airfoil_small():
+    """Hybrid NACA 0012 O-mesh, ~8k cells, colour-blocked numbering."""
+    return mesh.prepare(mesh.airfoil_hybrid(ni=64, dy1=1e-4, rfar=20.0))[0]
+
+
+def perturbed(m, cc, flow_fs, amp=0.05, seed=0):
+    """Smooth perturbation of the free stream (so gradients and the limiter
+    are exercised), as AoS U[nc,4]."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.5, 2.0, size=(4, 2))
+    ph = rng.uniform(0, 2 * np.pi, size=4)
+    x, y = cc[:, 0], cc[:, 1]
+    U = np.tile(np.asarray(flow_fs, dtype=np.float64), (m.nc, 1))
+    for j in range(4):
+        scale = 1.0 if j < 2 else max(abs(flow_fs[2]), 0.1)
+        U[:, j] = U[:, j] + amp * scale * np.sin(a[j, 0] * x + a[j, 1] * y + ph[j])
+    return U
+
+
+def relerr(a, b):
+    """max |a-b| / max |b| per column (whole array for 1-D)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.ndim == 1:
+        return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    return np.max(np.abs(a - b), axis=0) / np.maximum(np.max(np.abs(b), axis=0), 1e-300)
+
+
+def exact_fraction(a, b):
+    return float(np.mean(np.asarray(a) == np.asarray(b)))

@@ -1,0 +1,33 @@
+"""Builds libusg.so (gfx950 HIP kernels + C ABI) in-tree with hipcc."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libusg.so")
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("solver.hip", "geometry.cpp")]
+HEADERS = [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "geometry.h")] + [
+    os.path.join(os.path.dirname(HERE), "include", "usg.h")]
+
+# -ffp-contract=off: no FMA contraction, so FP64 kernels keep the reference's
+# rounding; packed FP32 instructions are disabled (their results were not
+# reliable on this hardware pool).
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    cmd = [hipcc, *FLAGS, *SOURCES, "-o", LIB + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print(r.stderr)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

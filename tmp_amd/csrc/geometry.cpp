@@ -1,0 +1,233 @@
+// Host geometry precompute, restated from the reference's serial reader and
+// geometry types so that every derived value has the reference's bits:
+//   cell vertex order   UniTioga/CellBrick.cpp:13-136 (Brick4Cobalt2D::insert)
+//   per-cell face list  reader/UnstructReader.h:233-247 (push_edge in face order)
+//   cell volume         mathTools/cellVolume.cpp:6-78
+//   cell centre         toolBox/cell3d_int.h:100-108 (vertex mean)
+//   face n, A, centre   toolBox/edge3d_int.h:106-115,200-239
+//   wall distance       UnstructSolver/UnstructBlock.h:265-298 (projected)
+#include "geometry.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace usg {
+namespace {
+
+struct Brick {
+  int nodes[4] = {-1, -1, -1, -1};
+  int size = 0;
+  bool closed = false;
+  void insert(const int *start) {
+    if (closed) return;
+    if (size == 0) {
+      nodes[0] = start[0];
+      nodes[1] = start[1];
+      size = 2;
+      return;
+    }
+    int idx[8];
+    int k = 0;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 2; j++)
+        if (nodes[i] == start[j]) {
+          idx[k++] = i;
+          idx[k++] = j;
+        }
+    if (k == 0) return;  // opposite side of a quadrilateral
+    if (k == 2) {
+      if (idx[0] < 2)
+        nodes[3 - idx[0]] = start[1 - idx[1]];
+      else
+        nodes[5 - idx[0]] = start[1 - idx[1]];
+      size += 1;
+      if (size == 4) closed = true;
+      return;
+    }
+    if (size == 3 && nodes[2] == -1 && nodes[3] != -1) {
+      nodes[2] = nodes[3];
+      nodes[3] = -1;
+    }
+    closed = true;
+  }
+};
+
+double tri_area(const double *p1, const double *p2, const double *p3) {
+  double a = 0.5 * ((p1[0] - p2[0]) * (p1[1] + p2[1]) + (p2[0] - p3[0]) * (p2[1] + p3[1]) +
+                    (p3[0] - p1[0]) * (p3[1] + p1[1]));
+  return a < 0 ? -a : a;
+}
+
+double cross_z(double ax, double ay, double bx, double by) { return ax * by - ay * bx; }
+
+double volume(double xv[4][2], int nv) {
+  if (nv == 3) return tri_area(xv[0], xv[1], xv[2]);
+  if (nv != 4) return -1;
+  double ABx = xv[1][0] - xv[0][0], ABy = xv[1][1] - xv[0][1];
+  double ACx = xv[2][0] - xv[0][0], ACy = xv[2][1] - xv[0][1];
+  double ADx = xv[3][0] - xv[0][0], ADy = xv[3][1] - xv[0][1];
+  double ABcAC = cross_z(ABx, ABy, ACx, ACy);
+  double ABcAD = cross_z(ABx, ABy, ADx, ADy);
+  double area = 0;
+  if (ABcAC * ABcAD > 0) {
+    double ACcAD = cross_z(ACx, ACy, ADx, ADy);
+    if (ACcAD * ABcAD > 0) {
+      area += tri_area(xv[0], xv[2], xv[1]);
+      area += tri_area(xv[0], xv[2], xv[3]);
+    } else {
+      area += tri_area(xv[0], xv[3], xv[1]);
+      area += tri_area(xv[0], xv[3], xv[2]);
+    }
+  } else {
+    area += tri_area(xv[0], xv[1], xv[2]);
+    area += tri_area(xv[0], xv[1], xv[3]);
+  }
+  return area;
+}
+
+inline double dot2(double a0, double a1, double b0, double b1) {
+  double r = 0;
+  r += a0 * b0;
+  r += a1 * b1;
+  return r;
+}
+
+void waves(HostMesh &m) {
+  const int nc = m.nc;
+  auto other = [&](int k, int face) { return m.flc[face] == k ? m.frc[face] : m.flc[face]; };
+  std::vector<int> lev(nc, 0);
+  int nl = 0;
+  for (int k = 0; k < nc; k++) {
+    int lv = 0;
+    for (int f = 0; f < m.cnf[k]; f++) {
+      int o = other(k, m.cface[4 * k + f]);
+      if (o >= 0 && o < k) lv = std::max(lv, lev[o] + 1);
+    }
+    lev[k] = lv;
+    nl = std::max(nl, lv + 1);
+  }
+  m.fwd_off.assign(nl + 1, 0);
+  for (int k = 0; k < nc; k++) m.fwd_off[lev[k] + 1]++;
+  for (int l = 0; l < nl; l++) m.fwd_off[l + 1] += m.fwd_off[l];
+  m.fwd_cells.assign(nc, 0);
+  {
+    std::vector<int> pos(m.fwd_off.begin(), m.fwd_off.end() - 1);
+    for (int k = 0; k < nc; k++) m.fwd_cells[pos[lev[k]]++] = k;
+  }
+  std::fill(lev.begin(), lev.end(), 0);
+  nl = 0;
+  for (int k = nc - 1; k >= 0; k--) {
+    int lv = 0;
+    for (int f = 0; f < m.cnf[k]; f++) {
+      int o = other(k, m.cface[4 * k + f]);
+      if (o > k) lv = std::max(lv, lev[o] + 1);
+    }
+    lev[k] = lv;
+    nl = std::max(nl, lv + 1);
+  }
+  m.bwd_off.assign(nl + 1, 0);
+  for (int k = 0; k < nc; k++) m.bwd_off[lev[k] + 1]++;
+  for (int l = 0; l < nl; l++) m.bwd_off[l + 1] += m.bwd_off[l];
+  m.bwd_cells.assign(nc, 0);
+  {
+    std::vector<int> pos(m.bwd_off.begin(), m.bwd_off.end() - 1);
+    for (int k = 0; k < nc; k++) m.bwd_cells[pos[lev[k]]++] = k;
+  }
+}
+
+}  // namespace
+
+std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,
+                       const int *flc, const int *frc, int nc) {
+  if (nn <= 0 || nf <= 0 || nc <= 0 || !xy || !fnode || !flc || !frc) return "empty mesh";
+  m.nn = nn;
+  m.nf = nf;
+  m.nc = nc;
+  m.xy.assign(xy, xy + 2 * (size_t)nn);
+  m.fnode.assign(fnode, fnode + 2 * (size_t)nf);
+  m.flc.assign(flc, flc + nf);
+  m.frc.assign(frc, frc + nf);
+  for (int k = 0; k < nf; k++) {
+    if (flc[k] < 0 || flc[k] >= nc) return "Left cell can't be negative";
+    if (frc[k] >= nc || (frc[k] < 0 && frc[k] != kWall && frc[k] != kFarfield))
+      return "Not a valid boundary type";
+    if (fnode[2 * k] < 0 || fnode[2 * k] >= nn || fnode[2 * k + 1] < 0 || fnode[2 * k + 1] >= nn)
+      return "face node out of range";
+  }
+  m.cnode.assign(4 * (size_t)nc, -1);
+  m.cnv.assign(nc, 0);
+  m.cface.assign(4 * (size_t)nc, kNoFace);
+  m.cnf.assign(nc, 0);
+  std::vector<Brick> br(nc);
+  for (int k = 0; k < nf; k++) {
+    br[flc[k]].insert(&fnode[2 * k]);
+    if (frc[k] >= 0) br[frc[k]].insert(&fnode[2 * k]);
+  }
+  for (int k = 0; k < nf; k++) {
+    int l = flc[k], r = frc[k];
+    if (m.cnf[l] >= 4 || (r >= 0 && m.cnf[r] >= 4)) return "cell with more than 4 faces";
+    m.cface[4 * l + m.cnf[l]++] = k;
+    if (r >= 0) m.cface[4 * r + m.cnf[r]++] = k;
+  }
+  m.vol.assign(nc, 0.0);
+  m.cc.assign(2 * (size_t)nc, 0.0);
+  for (int i = 0; i < nc; i++) {
+    int nv = br[i].size;
+    if (nv < 3 || nv > 4 || m.cnf[i] != nv) return "cell is not a triangle or quadrilateral";
+    m.cnv[i] = nv;
+    double xv[4][2];
+    for (int v = 0; v < nv; v++) {
+      m.cnode[4 * i + v] = br[i].nodes[v];
+      xv[v][0] = xy[2 * br[i].nodes[v]];
+      xv[v][1] = xy[2 * br[i].nodes[v] + 1];
+    }
+    m.vol[i] = volume(xv, nv);
+    double sx = 0, sy = 0;
+    for (int v = 0; v < nv; v++) {
+      sx = sx + xv[v][0];
+      sy = sy + xv[v][1];
+    }
+    m.cc[2 * i] = sx / nv;
+    m.cc[2 * i + 1] = sy / nv;
+  }
+  m.fn.assign(2 * (size_t)nf, 0.0);
+  m.fa.assign(nf, 0.0);
+  m.fc.assign(2 * (size_t)nf, 0.0);
+  m.wall_faces.clear();
+  for (int k = 0; k < nf; k++) {
+    const double *p0 = &xy[2 * fnode[2 * k]];
+    const double *p1 = &xy[2 * fnode[2 * k + 1]];
+    double dx = p1[0] - p0[0], dy = p1[1] - p0[1];
+    double L2 = 0;
+    L2 += dx * dx;
+    L2 += dy * dy;
+    L2 = std::sqrt(L2);
+    m.fa[k] = L2;
+    m.fn[2 * k] = dy / L2;
+    m.fn[2 * k + 1] = -dx / L2;
+    double sx = 0, sy = 0;
+    sx = sx + p0[0];
+    sy = sy + p0[1];
+    sx = sx + p1[0];
+    sy = sy + p1[1];
+    m.fc[2 * k] = sx / 2;
+    m.fc[2 * k + 1] = sy / 2;
+    if (frc[k] == kWall) m.wall_faces.push_back(k);
+  }
+  m.wdist.assign(nc, 0.0);
+  for (int i = 0; i < nc; i++) {
+    double d = 100000000000.0;
+    for (int k : m.wall_faces) {
+      double cx = m.cc[2 * i] - m.fc[2 * k];
+      double cy = m.cc[2 * i + 1] - m.fc[2 * k + 1];
+      double dist = std::fabs(dot2(cx, cy, m.fn[2 * k], m.fn[2 * k + 1]));
+      d = (dist < d) ? dist : d;
+    }
+    m.wdist[i] = d;
+  }
+  waves(m);
+  return std::string();
+}
+
+}  // namespace usg

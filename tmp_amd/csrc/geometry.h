@@ -1,0 +1,37 @@
+// Host-side mesh preparation for the gfx950 solver: the reference's one-time
+// geometry precompute (cell vertex order, volume, vertex-mean centre, face
+// normal/area/midpoint, projected wall distance) and the LU-SGS wave
+// schedule.  Plain C++17, no HIP.
+#pragma once
+#include <string>
+#include <vector>
+
+namespace usg {
+
+constexpr int kWall = -2;      // edge3d<2>::BoundaryType::WALL
+constexpr int kFarfield = -3;  // edge3d<2>::BoundaryType::FARFIELD
+constexpr int kNoFace = -2147483647 - 1;
+
+struct HostMesh {
+  int nn = 0, nf = 0, nc = 0;
+  std::vector<double> xy;     // [nn*2]
+  std::vector<int> fnode;     // [nf*2]
+  std::vector<int> flc, frc;  // [nf]
+  // derived
+  std::vector<int> cnode, cnv;       // [nc*4], [nc]
+  std::vector<int> cface, cnf;       // [nc*4] ascending, [nc]
+  std::vector<double> vol, cc;       // [nc], [nc*2]
+  std::vector<double> fn, fa, fc;    // [nf*2], [nf], [nf*2]
+  std::vector<double> wdist;         // [nc]
+  // LU-SGS waves: cells grouped by dependency level (ascending cell id inside)
+  std::vector<int> fwd_cells, fwd_off;
+  std::vector<int> bwd_cells, bwd_off;
+  std::vector<int> wall_faces;
+};
+
+// Validates and copies the mesh, then computes the derived geometry.
+// Returns an empty string on success, else the error text.
+std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,
+                       const int *flc, const int *frc, int nc);
+
+}  // namespace usg

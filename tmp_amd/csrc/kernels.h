@@ -1,0 +1,268 @@
+// Device-side data views and the per-face arithmetic shared by the gfx950
+// kernels.  Every expression keeps the reference's operand order so that,
+// built with -ffp-contract=off, FP64 results reproduce the serial CPU code
+// (paths relative to the reference checkout).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace usg {
+
+constexpr int kNoSlot = -2147483647 - 1;  // empty face slot of a triangle
+constexpr int kWallD = -2;
+constexpr int kFarD = -3;
+
+// Mesh, structure-of-arrays.  Cell c's face slot s lives at [s*nc + c]:
+//   cf  = face id f if c is the face's left cell, -(f+1) if its right cell,
+//         kNoSlot for an unused slot (triangles);
+//   cnb = neighbour cell across that face, or kWallD / kFarD.
+// Slots are in ascending face order (the reference's per-cell edge list).
+struct DevMesh {
+  int nc, nf;
+  const double *vol, *ccx, *ccy, *wdist;
+  const int *cf, *cnb;
+  const double *fnx, *fny, *fA, *fcx, *fcy;
+};
+
+// State, structure-of-arrays ([var*nc + cell]).
+struct DevState {
+  double *U, *W;        // current primitive / conservative
+  double *Un, *Wn;      // next iterate (double-buffered update)
+  double *grad;         // [8][nc] d(var)/dx, d(var)/dy interleaved per var: (2v+d)*nc
+  double *gradT;        // [2][nc]
+  double *lamc, *lamv;  // [nc]
+  double *mu;           // [nc] laminar viscosity at the cell
+  double *phi;          // [4][nc]
+  double *R;            // [4][nc]
+  double *dW1, *dW;     // [4][nc]
+  double *dt;           // [nc]
+  double *W0;           // [4][nc] RK5 stage origin
+  // SST
+  double *tU, *tW, *tUn, *tWn;  // [2][nc]
+  double *tgrad;                // [4][nc]: (2q+d)*nc
+  double *F1, *F2, *mut, *mutn; // [nc]
+  double *tR, *tdW1, *tdW;      // [2][nc]
+  int *err;                     // [2] invariant violation counters
+};
+
+struct Consts {
+  double fs[4];        // fs_primVar (rho, p, u, v) non-dimensional
+  double fs_c;         // fsnd_soundSpeed = sqrt(gamma)
+  double mach, re;     // fs_mach, fs_Re
+  double cc;           // 110.5 / fs_Temperature (Sutherland)
+  double pr, prt;      // 0.72, 0.9
+  double cfl;          // pseudo-time CFL
+  double fsnd_turb[2]; // free-stream (k, omega)
+  double fsnd_mut;     // free-stream eddy viscosity
+  double sqrt_gamma;   // sqrt(1.4)
+};
+
+#define USG_GAMMA 1.4
+
+__device__ __forceinline__ double dot2(double a0, double a1, double b0, double b1) {
+  double r = 0;
+  r += a0 * b0;
+  r += a1 * b1;
+  return r;
+}
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+
+// Far-field Riemann-invariant state, BoundaryStrategy.h:262-328 (inviscid) and
+// :120-187 (viscous; identical arithmetic).
+__device__ __forceinline__ void farfield_state(const Consts &k, const double *Ul, double n0, double n1,
+                                               double *Ue) {
+  const double Gamma = USG_GAMMA;
+  const double fsnd_density = k.fs[0], fsnd_pressure = k.fs[1];
+  const double fsu = k.fs[2], fsv = k.fs[3];
+  const double fsnd_soundSpeed = k.fs_c;
+  double Vnl = dot2(Ul[2], Ul[3], n0, n1);
+  double tl0 = Ul[2] - n0 * Vnl, tl1 = Ul[3] - n1 * Vnl;
+  double Vnr = dot2(fsu, fsv, n0, n1);
+  double tr0 = fsu - n0 * Vnr, tr1 = fsv - n1 * Vnr;
+  double rho_left = Ul[0], p_left = Ul[1];
+  if (fabs(Vnr) < fsnd_soundSpeed) {
+    double Vnb = (Vnl + Vnr + 2 * (sqrt(Gamma * p_left / rho_left) - fsnd_soundSpeed) / (Gamma - 1)) / 2;
+    double Cb = (Vnl - Vnr + 2 * (fsnd_soundSpeed + sqrt(Gamma * p_left / rho_left)) / (Gamma - 1)) *
+                (Gamma - 1) / 4;
+    double Sb, e0, e1;
+    if (Vnr >= 0) {
+      Sb = (p_left / pow(rho_left, Gamma));
+      e0 = tl0 + n0 * Vnb;
+      e1 = tl1 + n1 * Vnb;
+    } else {
+      Sb = (fsnd_pressure / pow(fsnd_density, Gamma));
+      e0 = tr0 + n0 * Vnb;
+      e1 = tr1 + n1 * Vnb;
+    }
+    Ue[0] = pow((Cb * Cb / (Gamma * Sb)), (1.0 / (Gamma - 1)));
+    Ue[1] = Ue[0] * Cb * Cb / Gamma;
+    Ue[2] = e0;
+    Ue[3] = e1;
+  } else if (Vnr >= 0) {
+    Ue[0] = Ul[0]; Ue[1] = Ul[1]; Ue[2] = Ul[2]; Ue[3] = Ul[3];
+  } else {
+    Ue[0] = fsnd_density; Ue[1] = fsnd_pressure; Ue[2] = fsu; Ue[3] = fsv;
+  }
+}
+
+// Face state U_edge of BoundaryStrategy::HandleInvicidBoundary /
+// HandleViscousBoundary for a boundary face of cell state Ul.
+// Returns false when the slip-wall check |t.n| > 1e-7 (:250-256) fails.
+template <bool VISC>
+__device__ __forceinline__ bool boundary_state(const Consts &k, int bc, const double *Ul, double n0,
+                                               double n1, double *Ue) {
+  if (bc == kWallD) {
+    Ue[0] = Ul[0];
+    Ue[1] = Ul[1];
+    if (VISC) {
+      Ue[2] = 0;
+      Ue[3] = 0;
+      return true;
+    }
+    double Vnl = dot2(Ul[2], Ul[3], n0, n1);
+    double t0 = Ul[2] - n0 * Vnl, t1 = Ul[3] - n1 * Vnl;
+    Ue[2] = t0;
+    Ue[3] = t1;
+    return !(dot2(t0, t1, n0, n1) > 1.0e-7);
+  }
+  farfield_state(k, Ul, n0, n1, Ue);
+  return true;
+}
+
+// Sutherland viscosity of LaminarFluxStrategy::computeMu (LaminarFluxStrategy.h:76-96)
+__device__ __forceinline__ double sutherland(const Consts &k, double T) {
+  return (1.0 + k.cc) / (T + k.cc) * pow(T, 1.5) * k.sqrt_gamma * k.mach / k.re;
+}
+
+// Venkatakrishnan limiter value for one face (Vankatakrishnan_Limiter.hpp:128-139)
+__device__ __forceinline__ double venkat(double d1max, double d1min, double delta2, double eps2) {
+  if (delta2 > 0)
+    return (d1max * d1max + eps2 + 2 * delta2 * d1max) /
+           (d1max * d1max + 2 * delta2 * delta2 + d1max * delta2 + eps2);
+  if (delta2 < 0)
+    return (d1min * d1min + eps2 + 2 * delta2 * d1min) /
+           (d1min * d1min + 2 * delta2 * delta2 + d1min * delta2 + eps2);
+  return 1;
+}
+
+// Interior HLLC flux (HLLCFluxStrategy.cpp:155-248).  Returns the p* check.
+__device__ __forceinline__ bool hllc(const double *UL, const double *UR, double n0, double n1, double area,
+                                     double *F) {
+  const double Gamma = USG_GAMMA;
+  double density_left = UL[0], pressure_left = UL[1];
+  double vl0 = UL[2], vl1 = UL[3];
+  double Vn_left = dot2(vl0, vl1, n0, n1);
+  double soundspeed_left = sqrt(Gamma * pressure_left / density_left);
+  double vl2 = dot2(vl0, vl1, vl0, vl1);
+  double enthalpy_left = pressure_left * Gamma / ((Gamma - 1) * density_left) + 0.5 * vl2;
+  double density_righ = UR[0], pressure_righ = UR[1];
+  double vr0 = UR[2], vr1 = UR[3];
+  double Vn_righ = dot2(vr0, vr1, n0, n1);
+  double soundspeed_righ = sqrt(Gamma * pressure_righ / density_righ);
+  double vr2 = dot2(vr0, vr1, vr0, vr1);
+  double enthalpy_righ = pressure_righ * Gamma / ((Gamma - 1) * density_righ) + 0.5 * vr2;
+  double efficient_left = sqrt(density_left) / (sqrt(density_left) + sqrt(density_righ));
+  double efficient_righ = 1 - efficient_left;
+  double enthalpy_tilda = efficient_left * enthalpy_left + efficient_righ * enthalpy_righ;
+  double Vn_tilda = efficient_left * Vn_left + efficient_righ * Vn_righ;
+  double soundspeed_tilda = sqrt((Gamma - 1) * (enthalpy_tilda - 0.5 * pow(Vn_tilda, 2.0)));
+  double Sl = smin(Vn_left - soundspeed_left, Vn_tilda - soundspeed_tilda);
+  double Sr = smax(Vn_tilda + soundspeed_tilda, Vn_righ + soundspeed_righ);
+  double Sm = (density_righ * Vn_righ * (Sr - Vn_righ) - density_left * Vn_left * (Sl - Vn_left) + pressure_left -
+               pressure_righ) /
+              (density_righ * (Sr - Vn_righ) - density_left * (Sl - Vn_left));
+  double p_star = density_left * (Vn_left - Sl) * (Vn_left - Sm) + pressure_left;
+  double p_star_scratch = density_righ * (Vn_righ - Sr) * (Vn_righ - Sm) + pressure_righ;
+  bool ok = fabs(p_star_scratch - p_star) < 1e-10;
+  if (Sl > 0.0) {
+    double rhoE = pressure_left / (Gamma - 1) + 0.5 * density_left * vl2;
+    F[0] = Vn_left * density_left * area;
+    F[1] = Vn_left * (rhoE + pressure_left) * area;
+    F[2] = (Vn_left * density_left * vl0 + pressure_left * n0) * area;
+    F[3] = (Vn_left * density_left * vl1 + pressure_left * n1) * area;
+  } else if (Sr < 0.0) {
+    double rhoE = pressure_righ / (Gamma - 1) + 0.5 * density_righ * vr2;
+    F[0] = Vn_righ * density_righ * area;
+    F[1] = Vn_righ * (rhoE + pressure_righ) * area;
+    F[2] = (Vn_righ * density_righ * vr0 + pressure_righ * n0) * area;
+    F[3] = (Vn_righ * density_righ * vr1 + pressure_righ * n1) * area;
+  } else {
+    double SRef, Vn_ref, density_ref, pressure_ref, v0, v1, vv;
+    if (Sm < 0.0) {
+      SRef = Sr; Vn_ref = Vn_righ; density_ref = density_righ; pressure_ref = pressure_righ;
+      v0 = vr0; v1 = vr1; vv = vr2;
+    } else {
+      SRef = Sl; Vn_ref = Vn_left; density_ref = density_left; pressure_ref = pressure_left;
+      v0 = vl0; v1 = vl1; vv = vl2;
+    }
+    double rhoE_ref = pressure_ref / (Gamma - 1) + 0.5 * density_ref * vv;
+    double f0 = (SRef - Vn_ref) * density_ref / (SRef - Sm);
+    double f1 = ((SRef - Vn_ref) * rhoE_ref - pressure_ref * Vn_ref + p_star * Sm) / (SRef - Sm);
+    double f2 = ((SRef - Vn_ref) * density_ref * v0 + (p_star - pressure_ref) * n0) / (SRef - Sm);
+    double f3 = ((SRef - Vn_ref) * density_ref * v1 + (p_star - pressure_ref) * n1) / (SRef - Sm);
+    F[0] = f0 * Sm * area;
+    F[1] = (f1 + p_star) * Sm * area;
+    F[2] = (f2 * Sm + p_star * n0) * area;
+    F[3] = (f3 * Sm + p_star * n1) * area;
+  }
+  return ok;
+}
+
+// Wall / far-field flux of HLLCFluxStrategy::computeFlux (HLLCFluxStrategy.cpp:126-154)
+__device__ __forceinline__ void boundary_flux(int bc, const double *Ue, double n0, double n1, double area,
+                                              double *F) {
+  const double Gamma = USG_GAMMA;
+  if (bc == kWallD) {
+    F[0] = 0 * area;
+    F[1] = 0 * area;
+    double P = Ue[1];
+    F[2] = P * n0 * area;
+    F[3] = P * n1 * area;
+  } else {
+    double D = Ue[0], P = Ue[1], v0 = Ue[2], v1 = Ue[3];
+    double rhoE = P / (Gamma - 1) + 0.5 * D * dot2(v0, v1, v0, v1);
+    double Vn = dot2(v0, v1, n0, n1);
+    F[0] = Vn * D * area;
+    F[1] = Vn * (rhoE + P) * area;
+    F[2] = (Vn * D * v0 + P * n0) * area;
+    F[3] = (Vn * D * v1 + P * n1) * area;
+  }
+}
+
+// ConservativeParam2ConvectiveFlux (TopologyHolderStrategy.cpp:129-141)
+__device__ __forceinline__ void cons_flux(const double *W, double n0, double n1, double *Fc) {
+  const double Gamma = USG_GAMMA;
+  double u = W[2] / W[0], v = W[3] / W[0];
+  double Vn = dot2(u, v, n0, n1);
+  double P = (Gamma - 1) * (W[1] - 0.5 * W[0] * dot2(u, v, u, v));
+  Fc[0] = W[0] * Vn;
+  Fc[1] = (W[1] + P) * Vn;
+  Fc[2] = W[2] * Vn + P * n0;
+  Fc[3] = W[3] * Vn + P * n1;
+}
+
+// UpdatePrimitiveVariables (TopologyHolderStrategy.cpp:89-109)
+__device__ __forceinline__ void prim_of(const double *W, double *U) {
+  const double Gamma = USG_GAMMA;
+  U[0] = W[0];
+  U[2] = W[2] / U[0];
+  U[3] = W[3] / U[0];
+  double VM = 0;
+  VM += U[2] * U[2];
+  VM += U[3] * U[3];
+  U[1] = (Gamma - 1) * (W[1] - 0.5 * U[0] * VM);
+}
+
+// initializeConsData (TopologyHolderStrategy.cpp:48-66)
+__device__ __forceinline__ void cons_of(const double *U, double *W) {
+  const double Gamma = USG_GAMMA;
+  double VM = 0;
+  VM += U[2] * U[2];
+  W[2] = U[0] * U[2];
+  VM += U[3] * U[3];
+  W[3] = U[0] * U[3];
+  W[0] = U[0];
+  W[1] = U[1] / (Gamma - 1) + 0.5 * U[0] * VM;
+}
+
+}  // namespace usg

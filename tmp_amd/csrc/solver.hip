@@ -1,0 +1,1301 @@
+// gfx950 kernels of the residual-and-update path and the context behind the
+// C ABI (include/usg.h).
+//
+// One thread per cell.  Every face loop is a gather over the cell's face slots
+// in ascending face order, which reproduces the reference's scatter order
+// (faces visited k = 0..nf-1, each adding to its left and right cell) without
+// floating-point atomics.  Kernels per LU-SGS iteration:
+//   k_grad    boundary states, Green-Gauss gradients (+grad T, +grad k/omega),
+//             spectral radii, Venkatakrishnan limiter, SST F1/F2
+//   k_flux    limited reconstruction + HLLC, viscous and SST fluxes, residual
+//   k_sweep   one launch per wave of the forward and the backward sweep; the
+//             last backward wave of each cell also writes the updated state
+//             into the second buffer (no separate update pass)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/usg.h"
+#include "geometry.h"
+#include "kernels.h"
+
+namespace usg {
+
+// ---------------------------------------------------------------------------
+// SST constants (SSTKWStrategy.h:13-29)
+// ---------------------------------------------------------------------------
+#define SST_BETA1 0.075
+#define SST_BETA2 0.0828
+#define SST_SK1 0.85
+#define SST_SK2 1.0
+#define SST_SW1 0.5
+#define SST_SW2 0.856
+#define SST_CW1 0.533
+#define SST_CW2 0.440
+#define SST_A1 0.31
+#define SST_BSTAR 0.09
+
+constexpr int kBlock = 256;
+
+template <int FLOW>
+struct Flow {
+  static constexpr bool visc = FLOW != USG_EULER;
+  static constexpr bool sst = FLOW == USG_SST;
+};
+
+__device__ __forceinline__ void load4(const double *a, int nc, int c, double *v) {
+  v[0] = a[c];
+  v[1] = a[nc + c];
+  v[2] = a[2 * nc + c];
+  v[3] = a[3 * nc + c];
+}
+__device__ __forceinline__ void store4(double *a, int nc, int c, const double *v) {
+  a[c] = v[0];
+  a[nc + c] = v[1];
+  a[2 * nc + c] = v[2];
+  a[3 * nc + c] = v[3];
+}
+
+// Face slot decode.
+struct Slot {
+  int f;      // face id
+  bool left;  // the cell is the face's left cell
+  int nb;     // neighbour cell or BC code
+};
+__device__ __forceinline__ bool get_slot(const DevMesh &m, int s, int c, Slot &o) {
+  int e = m.cf[s * m.nc + c];
+  if (e == kNoSlot) return false;
+  o.left = e >= 0;
+  o.f = o.left ? e : -e - 1;
+  o.nb = m.cnb[s * m.nc + c];
+  return true;
+}
+
+// face eddy viscosity of ComputeTurbulenceViscousCoefficient (SSTKWStrategy.h:755-776)
+__device__ __forceinline__ double face_mut(const Consts &k, const DevState &s, int nb, bool left, double mutc,
+                                           const double *Ue, double n0, double n1) {
+  if (nb >= 0) {
+    double mnb = s.mut[nb];
+    return left ? 0.5 * (mutc + mnb) : 0.5 * (mnb + mutc);
+  }
+  if (nb == kWallD) return 0.0;
+  double Vn = dot2(Ue[2], Ue[3], n0, n1);
+  return Vn > 0.0 ? mutc : k.fsnd_mut;
+}
+
+// ---------------------------------------------------------------------------
+// k_grad: boundary states + gradients + spectral radii + limiter (+ SST F1/F2)
+// Euler2D::updateEdgeValues/solveGradient/solveSpectralRadius (Euler2D.cpp:202-266),
+// ViscousFlow2D.cpp:174-252, Vankatakrishnan_Limiter.hpp:69-200,
+// LaminarFluxStrategy::computeMu (cells), SSTKWStrategy.h:616-722.
+// ---------------------------------------------------------------------------
+template <int FLOW>
+__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k) {
+  using F = Flow<FLOW>;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  const double Gamma = USG_GAMMA;
+  double Uc[4];
+  load4(s.U, nc, c, Uc);
+  const double V = m.vol[c];
+  double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double gT[2] = {0, 0};
+  double tg[4] = {0, 0, 0, 0};
+  double lc = 0, lv = 0;
+  double umax[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
+  double umin[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
+  double tUc[2] = {0, 0}, mutc = 0, muc = 0;
+  if (F::sst) {
+    tUc[0] = s.tU[c];
+    tUc[1] = s.tU[nc + c];
+    mutc = s.mut[c];
+  }
+  int bad = 0;
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+    double Ue[4], Unb[4];
+    if (o.nb >= 0) {
+      load4(s.U, nc, o.nb, Unb);
+      for (int j = 0; j < 4; j++) {
+        Ue[j] = o.left ? (Uc[j] + Unb[j]) / 2 : (Unb[j] + Uc[j]) / 2;
+        umax[j] = smax(umax[j], Unb[j]);
+        umin[j] = smin(umin[j], Unb[j]);
+      }
+    } else {
+      if (!boundary_state<F::visc>(k, o.nb, Uc, n0, n1, Ue)) bad = 1;
+    }
+    // Green-Gauss: + for the left cell, - for the right cell
+    if (o.left) {
+      for (int j = 0; j < 4; j++) {
+        g[2 * j] = g[2 * j] + n0 * (A * Ue[j]);
+        g[2 * j + 1] = g[2 * j + 1] + n1 * (A * Ue[j]);
+      }
+    } else {
+      for (int j = 0; j < 4; j++) {
+        g[2 * j] = g[2 * j] - n0 * (A * Ue[j]);
+        g[2 * j + 1] = g[2 * j + 1] - n1 * (A * Ue[j]);
+      }
+    }
+    double cs = sqrt(Gamma * Ue[1] / Ue[0]);
+    double Vn = dot2(Ue[2], Ue[3], n0, n1);
+    lc += (fabs(Vn) + cs) * A;
+    if (F::visc) {
+      double edgeT = Ue[1] / Ue[0];
+      if (o.left) {
+        gT[0] = gT[0] + n0 * (A * edgeT);
+        gT[1] = gT[1] + n1 * (A * edgeT);
+      } else {
+        gT[0] = gT[0] - n0 * (A * edgeT);
+        gT[1] = gT[1] - n1 * (A * edgeT);
+      }
+      double mue = sutherland(k, edgeT);
+      double mpr = mue / k.pr;
+      if (F::sst) mpr = mpr + face_mut(k, s, o.nb, o.left, mutc, Ue, n0, n1) / k.prt;
+      double Sv = smax(4 / (3.0 * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
+      lv += Sv / V;
+      if (F::sst) {
+        // ComputeFaceTValue (SSTKWStrategy.h:616-659)
+        double te[2];
+        if (o.nb >= 0) {
+          double t0 = s.tU[o.nb], t1 = s.tU[nc + o.nb];
+          te[0] = o.left ? 0.5 * (tUc[0] + t0) : 0.5 * (t0 + tUc[0]);
+          te[1] = o.left ? 0.5 * (tUc[1] + t1) : 0.5 * (t1 + tUc[1]);
+        } else if (o.nb == kWallD) {
+          double dis = m.wdist[c];
+          te[0] = 0.0;
+          te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
+        } else if (Vn > 0.0) {
+          te[0] = tUc[0];
+          te[1] = tUc[1];
+        } else {
+          te[0] = k.fsnd_turb[0];
+          te[1] = k.fsnd_turb[1];
+        }
+        for (int q = 0; q < 2; q++) {
+          if (o.left) {
+            tg[2 * q] = tg[2 * q] + n0 * (A * te[q]);
+            tg[2 * q + 1] = tg[2 * q + 1] + n1 * (A * te[q]);
+          } else {
+            tg[2 * q] = tg[2 * q] - n0 * (A * te[q]);
+            tg[2 * q + 1] = tg[2 * q + 1] - n1 * (A * te[q]);
+          }
+        }
+      }
+    }
+  }
+  for (int j = 0; j < 8; j++) g[j] = g[j] / V;
+  for (int j = 0; j < 8; j++) s.grad[j * nc + c] = g[j];
+  s.lamc[c] = lc;
+  if (F::visc) {
+    gT[0] = gT[0] / V;
+    gT[1] = gT[1] / V;
+    s.gradT[c] = gT[0];
+    s.gradT[nc + c] = gT[1];
+    s.lamv[c] = lv;
+    double T = Uc[1] / Uc[0];
+    muc = sutherland(k, T);
+    s.mu[c] = muc;
+  }
+  if (bad) atomicAdd(&s.err[1], 1);
+
+  // Venkatakrishnan limiter: eps^2 = 5^3 * V_left (integer 3/ndim == 1)
+  double d1max[4], d1min[4], ph[4] = {1.0, 1.0, 1.0, 1.0};
+  for (int j = 0; j < 4; j++) {
+    d1max[j] = umax[j] - Uc[j];
+    d1min[j] = umin[j] - Uc[j];
+  }
+  const double xc = m.ccx[c], yc = m.ccy[c];
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    double v0 = m.fcx[o.f] - xc, v1 = m.fcy[o.f] - yc;
+    double Vl = o.left ? V : m.vol[o.nb];
+    double eps2 = 125.0 * Vl;
+    for (int j = 0; j < 4; j++) {
+      double delta2 = dot2(g[2 * j], g[2 * j + 1], v0, v1);
+      ph[j] = smin(ph[j], venkat(d1max[j], d1min[j], delta2, eps2));
+    }
+  }
+  for (int j = 0; j < 4; j++) s.phi[j * nc + c] = ph[j];
+
+  if (F::sst) {
+    for (int q = 0; q < 4; q++) tg[q] = tg[q] / V;
+    for (int q = 0; q < 4; q++) s.tgrad[q * nc + c] = tg[q];
+    // ComputeSSTf1f2 (SSTKWStrategy.h:695-722)
+    double gradVal = dot2(tg[0], tg[1], tg[2], tg[3]);
+    double rho = Uc[0], dis = m.wdist[c];
+    double Kk = tUc[0], w = tUc[1];
+    double muL = muc;
+    double CD_kw = smax(2.0 * rho * SST_SW2 / w * gradVal, 1.0e-20);
+    double arg1 = smax(sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
+    arg1 = smin(arg1, 4.0 * rho * SST_SW2 * Kk / (CD_kw * dis * dis));
+    double arg2 = smax(2.0 * sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
+    s.F1[c] = tanh(arg1 * arg1 * arg1 * arg1);
+    s.F2[c] = tanh(arg2 * arg2);
+  }
+}
+
+// Limited reconstruction of one side: UL = (grad . (x_f - x_c)) * phi + U
+// (Vankatakrishnan_Limiter.hpp:118-125, HLLCFluxStrategy.h:56-65).
+__device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s, int cell, int f,
+                                            const double *Ucell, double *out) {
+  const int nc = m.nc;
+  double v0 = m.fcx[f] - m.ccx[cell], v1 = m.fcy[f] - m.ccy[cell];
+  for (int j = 0; j < 4; j++) {
+    double d2 = dot2(s.grad[(2 * j) * nc + cell], s.grad[(2 * j + 1) * nc + cell], v0, v1);
+    out[j] = d2 * s.phi[j * nc + cell] + Ucell[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_flux: residual of one cell.  HLLCFluxStrategy::computeFlux
+// (HLLCFluxStrategy.cpp:4-171), then the second face pass of
+// LaminarFluxStrategy::computeFlux (LaminarFluxStrategy.h:110-216), then the
+// SST residual (SSTKWStrategy.h:783-1013, source sign as deviation D1 of the
+// oracle: -Q*V).
+// ---------------------------------------------------------------------------
+template <int FLOW>
+__global__ __launch_bounds__(kBlock) void k_flux(const DevMesh m, const DevState s, const Consts k) {
+  using F = Flow<FLOW>;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  const double Gamma = USG_GAMMA;
+  double Uc[4];
+  load4(s.U, nc, c, Uc);
+  double Rc[4] = {0, 0, 0, 0};
+  int bad = 0;
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+    double Fx[4];
+    if (o.nb >= 0) {
+      double Unb[4], Ua[4], Ub[4];
+      load4(s.U, nc, o.nb, Unb);
+      if (o.left) {
+        reconstruct(m, s, c, o.f, Uc, Ua);
+        reconstruct(m, s, o.nb, o.f, Unb, Ub);
+      } else {
+        reconstruct(m, s, o.nb, o.f, Unb, Ua);
+        reconstruct(m, s, c, o.f, Uc, Ub);
+      }
+      if (!hllc(Ua, Ub, n0, n1, A, Fx)) bad = 1;
+    } else {
+      double Ue[4];
+      boundary_state<F::visc>(k, o.nb, Uc, n0, n1, Ue);
+      boundary_flux(o.nb, Ue, n0, n1, A, Fx);
+    }
+    if (o.left) {
+      for (int j = 0; j < 4; j++) Rc[j] += Fx[j];
+    } else {
+      for (int j = 0; j < 4; j++) Rc[j] -= Fx[j];
+    }
+  }
+  if (bad) atomicAdd(&s.err[0], 1);
+
+  if (F::visc) {
+    const double xc = m.ccx[c], yc = m.ccy[c];
+    const double mutc = F::sst ? s.mut[c] : 0.0;
+    double trc[2] = {0, 0};
+    double tUc[2] = {0, 0}, tgc[4] = {0, 0, 0, 0}, F1c = 0;
+    if (F::sst) {
+      tUc[0] = s.tU[c];
+      tUc[1] = s.tU[nc + c];
+      for (int q = 0; q < 4; q++) tgc[q] = s.tgrad[q * nc + c];
+      F1c = s.F1[c];
+      // SolveTotalTFlux source part (SSTKWStrategy.h:975-995)
+      double gux = s.grad[4 * nc + c], guy = s.grad[5 * nc + c];
+      double gvx = s.grad[6 * nc + c], gvy = s.grad[7 * nc + c];
+      double rho = Uc[0];
+      double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
+      double ddls1 = (guy + gvx) * (guy + gvx);
+      double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
+      double P = (ddls0 + ddls1) * mutc - ddls2;
+      double rdiv = dot2(tgc[0], tgc[1], tgc[2], tgc[3]);
+      double Cw = F1c * SST_CW1 + (1.0 - F1c) * SST_CW2;
+      double beta = F1c * SST_BETA1 + (1.0 - F1c) * SST_BETA2;
+      double kk = tUc[0], w = tUc[1];
+      double Q0 = P - SST_BSTAR * rho * kk * w;
+      double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
+      trc[0] = -(Q0 * m.vol[c]);
+      trc[1] = -(Q1 * m.vol[c]);
+    }
+    for (int sl = 0; sl < 4; sl++) {
+      Slot o;
+      if (!get_slot(m, sl, c, o)) break;
+      const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+      double Ue[4];
+      double Unb[4];
+      const int L = o.left ? c : o.nb, Rr = o.left ? o.nb : c;
+      if (o.nb >= 0) {
+        load4(s.U, nc, o.nb, Unb);
+        for (int j = 0; j < 4; j++) Ue[j] = o.left ? (Uc[j] + Unb[j]) / 2 : (Unb[j] + Uc[j]) / 2;
+      } else {
+        boundary_state<true>(k, o.nb, Uc, n0, n1, Ue);
+      }
+      double mue = sutherland(k, Ue[1] / Ue[0]);
+      double mutf = F::sst ? face_mut(k, s, o.nb, o.left, mutc, Ue, n0, n1) : 0.0;
+      double mu_at_face = F::sst ? mue + mutf : mue;
+      double mpr = mue / k.pr;
+      if (F::sst) mpr = mpr + mutf / k.prt;
+      double efficient = 2.0 / 3.0 * mu_at_face;
+      double gE[3][2];
+      if (o.nb >= 0) {
+        const double *UL = o.left ? Uc : Unb;
+        const double *UR = o.left ? Unb : Uc;
+        double l2r0 = m.ccx[Rr] - m.ccx[L], l2r1 = m.ccy[Rr] - m.ccy[L];
+        double l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
+        for (int j = 0; j < 2; j++) {
+          double a0 = (s.grad[(4 + 2 * j) * nc + L] + s.grad[(4 + 2 * j) * nc + Rr]) * 0.5;
+          double a1 = (s.grad[(5 + 2 * j) * nc + L] + s.grad[(5 + 2 * j) * nc + Rr]) * 0.5;
+          double sc = (UR[2 + j] - UL[2 + j] - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+          gE[j][0] = a0 + l2r0 * sc;
+          gE[j][1] = a1 + l2r1 * sc;
+        }
+        double a0 = (s.gradT[L] + s.gradT[Rr]) * 0.5;
+        double a1 = (s.gradT[nc + L] + s.gradT[nc + Rr]) * 0.5;
+        double rightT = UR[1] / UR[0];
+        double leftT = UL[1] / UL[0];
+        double sc = (rightT - leftT - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+        gE[2][0] = a0 + l2r0 * sc;
+        gE[2][1] = a1 + l2r1 * sc;
+      } else {
+        gE[0][0] = s.grad[4 * nc + c];
+        gE[0][1] = s.grad[5 * nc + c];
+        gE[1][0] = s.grad[6 * nc + c];
+        gE[1][1] = s.grad[7 * nc + c];
+        if (o.nb == kWallD) {
+          gE[2][0] = 0;
+          gE[2][1] = 0;
+        } else {
+          gE[2][0] = s.gradT[c];
+          gE[2][1] = s.gradT[nc + c];
+        }
+      }
+      double taoxx = efficient * (2.0 * gE[0][0] - gE[1][1]);
+      double taoyy = efficient * (2.0 * gE[1][1] - gE[0][0]);
+      double taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
+      double qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
+      double qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
+      double Fv[4];
+      Fv[0] = 0.0;
+      Fv[1] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
+      Fv[2] = (taoxx * n0 + taoxy * n1) * A;
+      Fv[3] = (taoxy * n0 + taoyy * n1) * A;
+      if (o.left) {
+        for (int j = 0; j < 4; j++) Rc[j] -= Fv[j];
+      } else {
+        for (int j = 0; j < 4; j++) Rc[j] += Fv[j];
+      }
+      if (F::sst) {
+        // face turbulence values (SSTKWStrategy.h:616-659)
+        double Vn = dot2(Ue[2], Ue[3], n0, n1);
+        double te[2], tnb[2] = {0, 0};
+        if (o.nb >= 0) {
+          tnb[0] = s.tU[o.nb];
+          tnb[1] = s.tU[nc + o.nb];
+          te[0] = o.left ? 0.5 * (tUc[0] + tnb[0]) : 0.5 * (tnb[0] + tUc[0]);
+          te[1] = o.left ? 0.5 * (tUc[1] + tnb[1]) : 0.5 * (tnb[1] + tUc[1]);
+        } else if (o.nb == kWallD) {
+          double dis = m.wdist[c];
+          te[0] = 0.0;
+          te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
+        } else if (Vn > 0.0) {
+          te[0] = tUc[0];
+          te[1] = tUc[1];
+        } else {
+          te[0] = k.fsnd_turb[0];
+          te[1] = k.fsnd_turb[1];
+        }
+        // convective (SolveConvectiveTFlux :783-831)
+        double Fc[2];
+        if (o.nb >= 0) {
+          double Vp = 0.5 * (Vn + fabs(Vn));
+          double Vs = 0.5 * (Vn - fabs(Vn));
+          const double *tl = o.left ? tUc : tnb;
+          const double *tr = o.left ? tnb : tUc;
+          double rl = o.left ? Uc[0] : Unb[0], rr = o.left ? Unb[0] : Uc[0];
+          for (int q = 0; q < 2; q++) Fc[q] = tl[q] * rl * Vp * A + tr[q] * rr * Vs * A;
+        } else {
+          for (int q = 0; q < 2; q++) Fc[q] = Vn * Ue[0] * te[q] * A;
+        }
+        // diffusive (SolveViscousTFlux :834-920)
+        double face_f1;
+        double gq[2][2];
+        if (o.nb >= 0) {
+          double l0 = m.ccx[Rr] - m.ccx[L], l1 = m.ccy[Rr] - m.ccy[L];
+          const double *lt = o.left ? tUc : tnb;
+          const double *rt = o.left ? tnb : tUc;
+          double f1l = o.left ? F1c : s.F1[o.nb], f1r = o.left ? s.F1[o.nb] : F1c;
+          face_f1 = 0.5 * (f1l + f1r);
+          for (int q = 0; q < 2; q++) {
+            gq[q][0] = (s.tgrad[(2 * q) * nc + L] + s.tgrad[(2 * q) * nc + Rr]) * 0.5;
+            gq[q][1] = (s.tgrad[(2 * q + 1) * nc + L] + s.tgrad[(2 * q + 1) * nc + Rr]) * 0.5;
+          }
+          double l2 = dot2(l0, l1, l0, l1);
+          for (int q = 0; q < 2; q++) {
+            gq[q][0] = gq[q][0] + ((rt[q] - lt[q]) - (gq[q][0] * l0 + gq[q][1] * l1)) * l0 / l2;
+            gq[q][1] = gq[q][1] + ((rt[q] - lt[q]) - (gq[q][0] * l0 + gq[q][1] * l1)) * l1 / l2;
+          }
+        } else {
+          face_f1 = F1c;
+          for (int q = 0; q < 2; q++) {
+            gq[q][0] = tgc[2 * q];
+            gq[q][1] = tgc[2 * q + 1];
+          }
+        }
+        double sigma_k = face_f1 * SST_SK1 + (1 - face_f1) * SST_SK2;
+        double sigma_w = face_f1 * SST_SW1 + (1 - face_f1) * SST_SW2;
+        double txx_k = (mue + sigma_k * mutf) * gq[0][0];
+        double tyy_k = (mue + sigma_k * mutf) * gq[0][1];
+        double txx_w = (mue + sigma_w * mutf) * gq[1][0];
+        double tyy_w = (mue + sigma_w * mutf) * gq[1][1];
+        double Fv0 = txx_k * n0 + tyy_k * n1;
+        double Fv1 = txx_w * n0 + tyy_w * n1;
+        Fv0 *= A;
+        Fv1 *= A;
+        if (o.left) {
+          trc[0] += Fc[0] - Fv0;
+          trc[1] += Fc[1] - Fv1;
+        } else {
+          trc[0] -= Fc[0] - Fv0;
+          trc[1] -= Fc[1] - Fv1;
+        }
+      }
+    }
+    if (F::sst) {
+      s.tR[c] = trc[0];
+      s.tR[nc + c] = trc[1];
+    }
+  }
+  store4(s.R, nc, c, Rc);
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep: one wave of the LU-SGS forward (FWD) or backward sweep.
+// LUSGSIntegrator::SolveForwardSweep/SolveBackwardSweep (LUSGS_Integrator.cpp:492-561),
+// SolveDF/SolveViscousFlux (TopologyHolderStrategy.cpp:143-323), SolveTime
+// (Euler2D.h:67-77, ViscousFlow2D.h:223-242), SolveDiag (:190-215); SST
+// SolveDiag/SolveDF (SSTKWStrategy.h:170-192,251-417, deviations D2/D4).
+// The backward wave also applies UpdatePrimitiveVariable
+// (LUSGS_Integrator.cpp:242-255) into the second state buffer.
+// ---------------------------------------------------------------------------
+template <int FLOW, bool FWD>
+__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
+                                                 const int *__restrict__ cells, int count) {
+  using F = Flow<FLOW>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int c = cells[i];
+  const int nc = m.nc;
+  const double Gamma = USG_GAMMA, OMEGAN = 1.5;
+  const double V = m.vol[c];
+  const double lamc = s.lamc[c];
+  const double lamv = F::visc ? s.lamv[c] : 0.0;
+  double dtc;
+  if (F::sst)
+    dtc = k.cfl * V / (lamc + 4.0 * lamv);
+  else
+    dtc = k.cfl * V / (lamc);
+  double diag = V / dtc + 0.5 * OMEGAN * lamc;
+  if (F::visc) diag += lamv;
+  double mprc = 0;
+  if (F::visc) {
+    mprc = s.mu[c] / k.pr;
+    if (F::sst) mprc = mprc + s.mut[c] / k.prt;
+  }
+  const double *dsrc = FWD ? s.dW1 : s.dW;
+  const double *tsrc = FWD ? s.tdW1 : s.tdW;
+  double dFi[4] = {0, 0, 0, 0};
+  double tFi[2] = {0, 0};
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    if (o.nb < 0) continue;
+    if (FWD ? !(o.nb < c) : !(o.nb > c)) continue;
+    const int nb = o.nb;
+    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+    double W[4], Wp[4], dW[4], Fc[4], Fcp[4];
+    for (int j = 0; j < 4; j++) {
+      double W0 = s.W[j * nc + nb];
+      dW[j] = dsrc[j * nc + nb];
+      Wp[j] = W0 + dW[j];
+      W[j] = W0;
+    }
+    cons_flux(W, n0, n1, Fc);
+    cons_flux(Wp, n0, n1, Fcp);
+    double Ua[4];
+    load4(s.U, nc, nb, Ua);
+    double Vn = dot2(Ua[2], Ua[3], n0, n1);
+    double cs = sqrt(Gamma * Ua[1] / Ua[0]);
+    double rA = OMEGAN * (fabs(Vn) + cs);
+    if (F::visc) {
+      double cl0 = m.ccx[nb] - m.ccx[c], cl1 = m.ccy[nb] - m.ccy[c];
+      rA += smax(4 / (3 * Ua[0]), Gamma / Ua[0]) * (mprc) / sqrt(dot2(cl0, cl1, cl0, cl1));
+    }
+    for (int j = 0; j < 4; j++) {
+      double DFc = o.left ? (Fcp[j] - Fc[j]) : (Fc[j] - Fcp[j]);
+      double Fv = rA * dW[j];
+      dFi[j] += 0.5 * (DFc - Fv) * A;
+    }
+    if (F::sst) {
+      double tw[2], tdw[2];
+      for (int q = 0; q < 2; q++) {
+        tw[q] = s.tW[q * nc + nb];
+        tdw[q] = tsrc[q * nc + nb];
+      }
+      double u = W[2] / W[0], v = W[3] / W[0];
+      double vn = dot2(u, v, n0, n1);
+      double up = Wp[2] / Wp[0], vp = Wp[3] / Wp[0];
+      double vnp = dot2(up, vp, n0, n1);
+      for (int q = 0; q < 2; q++) {
+        double twp = tw[q] + tdw[q];
+        double fc = tw[q] * vn, fcp = twp * vnp;
+        double DFc = o.left ? (fcp - fc) : (fc - fcp);
+        double Fv = rA * tdw[q];
+        tFi[q] += 0.5 * (DFc - Fv) * A;
+      }
+    }
+  }
+  double tD[2] = {0, 0};
+  if (F::sst) {
+    // SSTkomegaModel::SolveDiag
+    double w = s.tU[nc + c];
+    double f1 = s.F1[c];
+    double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
+    tD[0] = diagt + 2.0 * SST_BSTAR * w * V;
+    double gradVal = dot2(s.tgrad[c], s.tgrad[nc + c], s.tgrad[2 * nc + c], s.tgrad[3 * nc + c]);
+    double beta = f1 * SST_BETA1 + (1 - f1) * SST_BETA2;
+    tD[1] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - f1) * gradVal / (w * w));
+  }
+  if (FWD) {
+    for (int j = 0; j < 4; j++) s.dW1[j * nc + c] = (-s.R[j * nc + c] - dFi[j]) / diag;
+    if (F::sst)
+      for (int q = 0; q < 2; q++) s.tdW1[q * nc + c] = (-s.tR[q * nc + c] - tFi[q]) / tD[q];
+  } else {
+    double dW[4], Wn[4], Un[4];
+    for (int j = 0; j < 4; j++) {
+      dW[j] = s.dW1[j * nc + c] - dFi[j] / diag;
+      s.dW[j * nc + c] = dW[j];
+      Wn[j] = s.W[j * nc + c] + dW[j];
+    }
+    prim_of(Wn, Un);
+    store4(s.Wn, nc, c, Wn);
+    store4(s.Un, nc, c, Un);
+    if (F::sst) {
+      double tdw[2], tw[2];
+      for (int q = 0; q < 2; q++) {
+        tdw[q] = s.tdW1[q * nc + c] - tFi[q] / tD[q];
+        s.tdW[q * nc + c] = tdw[q];
+        tw[q] = s.tW[q * nc + c] + tdw[q];
+        s.tWn[q * nc + c] = tw[q];
+      }
+      // SSTkomegaModel::UpdatePrimitiveVariables (SSTKWStrategy.h:194-223)
+      double rho = Un[0];
+      double tu[2] = {tw[0] / rho, tw[1] / rho};
+      // floor at 1e-4 of the free stream (the clip kept commented out at
+      // SSTKWStrategy.h:206-217); a NaN is floored too
+      for (int q = 0; q < 2; q++) {
+        double fl = k.fsnd_turb[q] * 1.0e-4;
+        if (!(tu[q] >= fl)) {
+          tu[q] = fl;
+          tw[q] = rho * fl;
+          s.tWn[q * nc + c] = tw[q];
+        }
+      }
+      s.tUn[c] = tu[0];
+      s.tUn[nc + c] = tu[1];
+      double curl = fabs(s.grad[6 * nc + c] - s.grad[5 * nc + c]);
+      double h = smax(SST_A1 * tw[1] / rho, s.F2[c] * curl);
+      s.mutn[c] = SST_A1 * tw[0] / h;
+    }
+  }
+}
+
+// RK5 stage (RungeKuttaStrategy.h:39-72): W_now = W0 + alpha*dt*(-R)/V ; U = prim(W_now)
+__global__ __launch_bounds__(kBlock) void k_rk_stage(const DevMesh m, const DevState s, double cfl, double alpha,
+                                                     int stage, int visc_sst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  const double V = m.vol[c];
+  if (stage == 0) {
+    double dtc = visc_sst ? cfl * V / (s.lamc[c] + 4.0 * s.lamv[c]) : cfl * V / (s.lamc[c]);
+    s.dt[c] = dtc;
+  }
+  const double dtc = s.dt[c];
+  double Wn[4], Un[4];
+  for (int j = 0; j < 4; j++) Wn[j] = s.W0[j * nc + c] + alpha * dtc * (-s.R[j * nc + c]) / V;
+  prim_of(Wn, Un);
+  store4(s.W, nc, c, Wn);
+  store4(s.U, nc, c, Un);
+}
+
+// initializeConsData of the current U into W (and W0)
+__global__ __launch_bounds__(kBlock) void k_cons(const DevMesh m, const DevState s, double *Wdst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  double U[4], W[4];
+  load4(s.U, nc, c, U);
+  cons_of(U, W);
+  store4(Wdst, nc, c, W);
+}
+
+// SSTkomegaModel::initializeConsData (SSTKWStrategy.h:97-109)
+__global__ __launch_bounds__(kBlock) void k_tcons(const DevMesh m, const DevState s) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  double rho = s.U[c];
+  s.tW[c] = s.tU[c] * rho;
+  s.tW[nc + c] = s.tU[nc + c] * rho;
+}
+
+// Free-stream cold start (Euler2D.cpp:153-171, SSTKWStrategy.h:582-613)
+__global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const DevState s, const Consts k, int sst) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc;
+  if (c >= nc) return;
+  for (int j = 0; j < 4; j++) s.U[j * nc + c] = k.fs[j];
+  if (sst) {
+    s.tU[c] = k.fsnd_turb[0];
+    s.tU[nc + c] = k.fsnd_turb[1];
+    s.mut[c] = k.fsnd_mut;
+  }
+}
+
+// ||R_rho / V||^2 partial sums, fixed-order tree per block (deterministic).
+__global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const double *R, double *partial) {
+  __shared__ double sh[kBlock];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0;
+  if (c < m.nc) {
+    double r = R[c] / m.vol[c];
+    v = r * r;
+  }
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_norm_final(const double *partial, int n, double *out) {
+  __shared__ double sh[kBlock];
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sqrt(sh[0]);
+}
+
+// Wall force: sum over wall faces of (p n A) from the left cell's boundary state.
+__global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const int *wall_faces,
+                                                   const int *wall_cells, int nw, double *out2) {
+  __shared__ double sx[kBlock], sy[kBlock];
+  double fx = 0, fy = 0;
+  for (int i = threadIdx.x; i < nw; i += kBlock) {
+    int f = wall_faces[i], c = wall_cells[i];
+    double p = s.U[m.nc + c];
+    fx = fx + p * m.fnx[f] * m.fA[f];
+    fy = fy + p * m.fny[f] * m.fA[f];
+  }
+  sx[threadIdx.x] = fx;
+  sy[threadIdx.x] = fy;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      sx[threadIdx.x] = sx[threadIdx.x] + sx[threadIdx.x + w];
+      sy[threadIdx.x] = sy[threadIdx.x] + sy[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out2[0] = sx[0];
+    out2[1] = sy[0];
+  }
+}
+
+// Donor-stencil apply, ROW layout (UniTioga/dataUpdate.cpp:24-72,115-133):
+// receptor value = sum_m w_m * q[donor_m], written to the receptor.
+// Reads and writes are split over two buffers' roles by the caller: donors are
+// never receptors within one apply (TIOGA's fringe/donor exclusivity).
+__global__ __launch_bounds__(kBlock) void k_stencil(double *q, int nvar, int nrecv, const int *recv, const int *nw,
+                                                    const int *donor, const double *w) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nrecv * nvar) return;
+  const int r = t / nvar, kv = t - r * nvar;
+  double acc = 0;
+  const int n = nw[r];
+  for (int mm = 0; mm < n; mm++) acc += q[(long)donor[4 * r + mm] * nvar + kv] * w[4 * r + mm];
+  q[(long)recv[r] * nvar + kv] = acc;
+}
+
+}  // namespace usg
+
+// ===========================================================================
+// Host context
+// ===========================================================================
+using namespace usg;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                          \
+  do {                                                                                     \
+    hipError_t e_ = (x);                                                                   \
+    if (e_ != hipSuccess) return fail(USG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t n = 0;
+};
+
+struct usg_ctx {
+  HostMesh hm;
+  usg_params prm{};
+  Consts k{};
+  DevMesh dm{};
+  DevState ds{};
+  hipStream_t own = nullptr, stream = nullptr;
+  std::vector<void *> allocs;
+  int *d_fwd = nullptr, *d_bwd = nullptr;
+  int *d_wall_faces = nullptr, *d_wall_cells = nullptr;
+  double *d_partial = nullptr, *d_norms = nullptr, *d_force = nullptr;
+  int nblk = 0;
+  int norms_cap = 0;
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  size_t ev_used = 0;
+  double fs_aoa_rad = 0;
+  bool have_residual = false;
+};
+
+template <typename T>
+static int dalloc(usg_ctx *c, T **p, size_t n) {
+  void *q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+  HIPCHK(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), c->stream));
+  c->allocs.push_back(q);
+  *p = (T *)q;
+  return USG_OK;
+}
+
+template <typename T>
+static int upload(usg_ctx *c, T **p, const T *src, size_t n) {
+  int rc = dalloc(c, p, n);
+  if (rc) return rc;
+  HIPCHK(hipMemcpyAsync(*p, src, n * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return USG_OK;
+}
+
+// set_fs_variable2 + update_nondim_variable (TopologyHolderStrategy.cpp:325-396)
+// with WrapSolver's density 1.225 and pressure 101325 (SolverWrapper.h:55).
+static void freestream(usg_ctx *c) {
+  const double R = 287.0, Gamma = 1.4, PI = 3.141593;
+  double fs_mach = c->prm.mach;
+  double fs_AOA = PI * c->prm.aoa_deg / 180;
+  double fs_density = 1.225, fs_pressure = 101325, fs_Re = c->prm.reynolds;
+  double fs_soundSpeed = std::sqrt(Gamma * fs_pressure / fs_density);
+  double fs_Temperature = fs_pressure / (R * fs_density);
+  double fs_velocity_magnitude = fs_soundSpeed * fs_mach;
+  double fs_mu = 1.458e-6 * std::pow(fs_Temperature, 1.5) / (fs_Temperature + 110.5);
+  double u = fs_velocity_magnitude * std::cos(fs_AOA);
+  double v = fs_velocity_magnitude * std::sin(fs_AOA);
+  c->k.fs[0] = fs_density / fs_density;
+  c->k.fs[1] = fs_pressure / fs_pressure;
+  c->k.fs[2] = u / (fs_soundSpeed / std::sqrt(Gamma));
+  c->k.fs[3] = v / (fs_soundSpeed / std::sqrt(Gamma));
+  c->k.fs_c = std::sqrt(Gamma);
+  c->k.sqrt_gamma = std::sqrt(Gamma);
+  c->k.mach = fs_mach;
+  c->k.re = fs_Re;
+  c->k.cc = 110.5 / fs_Temperature;
+  c->k.pr = 0.72;
+  c->k.prt = 0.9;
+  c->k.cfl = c->prm.cfl_lusgs;
+  c->fs_aoa_rad = fs_AOA;
+  // SSTkomegaModel::InitializeturbulenceVariable (SSTKWStrategy.h:582-613)
+  const double L = 1.0;
+  double fs_mut = fs_mu * 0.035;
+  c->k.fsnd_mut = fs_mut / fs_mu * std::sqrt(Gamma) * fs_mach / fs_Re;
+  double t0 = 1.5 * fs_mach * fs_soundSpeed * 0.01 * fs_mach * fs_soundSpeed * 0.01;
+  double t1 = fs_density * t0 / (0.035 * fs_mu);
+  c->k.fsnd_turb[0] = Gamma * t0 / (fs_soundSpeed * fs_soundSpeed);
+  c->k.fsnd_turb[1] = std::sqrt(Gamma) * L * t1 / fs_soundSpeed;
+}
+
+static int grid(int n) { return (n + kBlock - 1) / kBlock; }
+
+template <int FLOW>
+static void launch_residual(usg_ctx *c) {
+  const int nb = c->nblk;
+  hipLaunchKernelGGL(k_grad<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  if (c->timing) {
+    if (c->ev_used == c->ev.size()) {
+      hipEvent_t a, b;
+      hipEventCreate(&a);
+      hipEventCreate(&b);
+      c->ev.push_back({a, b});
+    }
+    hipEventRecord(c->ev[c->ev_used].first, c->stream);
+  }
+  hipLaunchKernelGGL(k_flux<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  if (c->timing) {
+    hipEventRecord(c->ev[c->ev_used].second, c->stream);
+    c->ev_used++;
+  }
+}
+
+static void launch_residual_any(usg_ctx *c) {
+  switch (c->prm.flow) {
+    case USG_EULER: launch_residual<USG_EULER>(c); break;
+    case USG_LAMINAR: launch_residual<USG_LAMINAR>(c); break;
+    default: launch_residual<USG_SST>(c); break;
+  }
+}
+
+template <int FLOW>
+static void launch_sweeps(usg_ctx *c) {
+  const auto &fo = c->hm.fwd_off;
+  for (size_t w = 0; w + 1 < fo.size(); w++) {
+    int cnt = fo[w + 1] - fo[w];
+    hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                       c->d_fwd + fo[w], cnt);
+  }
+  const auto &bo = c->hm.bwd_off;
+  for (size_t w = 0; w + 1 < bo.size(); w++) {
+    int cnt = bo[w + 1] - bo[w];
+    hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                       c->d_bwd + bo[w], cnt);
+  }
+}
+
+static void launch_norm(usg_ctx *c, double *dst) {
+  hipLaunchKernelGGL(k_norm_partial, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_partial);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, c->stream, c->d_partial, c->nblk, dst);
+}
+
+static void swap_state(usg_ctx *c) {
+  std::swap(c->ds.U, c->ds.Un);
+  std::swap(c->ds.W, c->ds.Wn);
+  if (c->prm.flow == USG_SST) {
+    std::swap(c->ds.tU, c->ds.tUn);
+    std::swap(c->ds.tW, c->ds.tWn);
+    std::swap(c->ds.mut, c->ds.mutn);
+  }
+}
+
+static int ensure_norms(usg_ctx *c, int n) {
+  if (n <= c->norms_cap) return USG_OK;
+  if (c->d_norms) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipFree(c->d_norms));
+    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void *)c->d_norms));
+  }
+  int rc = dalloc(c, &c->d_norms, n);
+  if (rc) return rc;
+  c->norms_cap = n;
+  return USG_OK;
+}
+
+static int check_err(usg_ctx *c) {
+  int h[2] = {0, 0};
+  HIPCHK(hipMemcpyAsync(h, c->ds.err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (h[0] || h[1]) {
+    HIPCHK(hipMemsetAsync(c->ds.err, 0, sizeof(h), c->stream));
+    return fail(USG_ERR_NUMERIC, h[0] ? "SOmething wrong at p star (HLLC |p*_L - p*_R| >= 1e-10)"
+                                      : "Something wrong with Vector computation (slip wall |t.n| > 1e-7)");
+  }
+  return USG_OK;
+}
+
+extern "C" {
+
+const char *usg_last_error(void) { return g_err.c_str(); }
+
+int usg_ctx_destroy(usg_ctx *c) {
+  if (!c) return USG_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto &e : c->ev) {
+    hipEventDestroy(e.first);
+    hipEventDestroy(e.second);
+  }
+  for (void *p : c->allocs) hipFree(p);
+  if (c->own) hipStreamDestroy(c->own);
+  delete c;
+  return USG_OK;
+}
+
+int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out) {
+  if (!mesh || !params || !out) return fail(USG_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (params->flow < USG_EULER || params->flow > USG_SST) return fail(USG_ERR_ARG, "Undefined Flow Type");
+  if (params->precision != USG_FP64 && params->precision != USG_MIXED) return fail(USG_ERR_ARG, "bad precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
+  usg_ctx *c = new usg_ctx();
+  c->prm = *params;
+  std::string e = build_mesh(c->hm, mesh->nn, mesh->xy, mesh->nf, mesh->fnode, mesh->flc, mesh->frc, mesh->nc);
+  if (!e.empty()) {
+    delete c;
+    return fail(USG_ERR_MESH, e);
+  }
+  freestream(c);
+  int rc = USG_OK;
+#define CK(x)                  \
+  do {                         \
+    rc = (x);                  \
+    if (rc) {                  \
+      usg_ctx_destroy(c);      \
+      return rc;               \
+    }                          \
+  } while (0)
+  if (hipStreamCreate(&c->own) != hipSuccess) {
+    delete c;
+    return fail(USG_ERR_HIP, "hipStreamCreate");
+  }
+  c->stream = c->own;
+  const HostMesh &h = c->hm;
+  const int nc = h.nc, nf = h.nf;
+  c->nblk = grid(nc);
+  std::vector<double> ccx(nc), ccy(nc), fnx(nf), fny(nf), fcx(nf), fcy(nf);
+  for (int i = 0; i < nc; i++) {
+    ccx[i] = h.cc[2 * i];
+    ccy[i] = h.cc[2 * i + 1];
+  }
+  for (int f = 0; f < nf; f++) {
+    fnx[f] = h.fn[2 * f];
+    fny[f] = h.fn[2 * f + 1];
+    fcx[f] = h.fc[2 * f];
+    fcy[f] = h.fc[2 * f + 1];
+  }
+  std::vector<int> cf(4 * (size_t)nc, kNoSlot), cnb(4 * (size_t)nc, kNoSlot);
+  for (int i = 0; i < nc; i++)
+    for (int s = 0; s < h.cnf[i]; s++) {
+      int f = h.cface[4 * i + s];
+      bool left = h.flc[f] == i;
+      cf[(size_t)s * nc + i] = left ? f : -(f + 1);
+      cnb[(size_t)s * nc + i] = left ? h.frc[f] : h.flc[f];
+    }
+  DevMesh &dm = c->dm;
+  dm.nc = nc;
+  dm.nf = nf;
+  CK(upload(c, (double **)&dm.vol, h.vol.data(), nc));
+  CK(upload(c, (double **)&dm.ccx, ccx.data(), nc));
+  CK(upload(c, (double **)&dm.ccy, ccy.data(), nc));
+  CK(upload(c, (double **)&dm.wdist, h.wdist.data(), nc));
+  CK(upload(c, (int **)&dm.cf, cf.data(), cf.size()));
+  CK(upload(c, (int **)&dm.cnb, cnb.data(), cnb.size()));
+  CK(upload(c, (double **)&dm.fnx, fnx.data(), nf));
+  CK(upload(c, (double **)&dm.fny, fny.data(), nf));
+  CK(upload(c, (double **)&dm.fA, h.fa.data(), nf));
+  CK(upload(c, (double **)&dm.fcx, fcx.data(), nf));
+  CK(upload(c, (double **)&dm.fcy, fcy.data(), nf));
+  CK(upload(c, &c->d_fwd, h.fwd_cells.data(), nc));
+  CK(upload(c, &c->d_bwd, h.bwd_cells.data(), nc));
+  std::vector<int> wc(h.wall_faces.size());
+  for (size_t i = 0; i < wc.size(); i++) wc[i] = h.flc[h.wall_faces[i]];
+  CK(upload(c, &c->d_wall_faces, h.wall_faces.data(), h.wall_faces.size()));
+  CK(upload(c, &c->d_wall_cells, wc.data(), wc.size()));
+  DevState &s = c->ds;
+  const bool visc = params->flow != USG_EULER, sst = params->flow == USG_SST;
+  CK(dalloc(c, &s.U, 4 * (size_t)nc));
+  CK(dalloc(c, &s.W, 4 * (size_t)nc));
+  CK(dalloc(c, &s.Un, 4 * (size_t)nc));
+  CK(dalloc(c, &s.Wn, 4 * (size_t)nc));
+  CK(dalloc(c, &s.grad, 8 * (size_t)nc));
+  CK(dalloc(c, &s.lamc, nc));
+  CK(dalloc(c, &s.phi, 4 * (size_t)nc));
+  CK(dalloc(c, &s.R, 4 * (size_t)nc));
+  CK(dalloc(c, &s.dW1, 4 * (size_t)nc));
+  CK(dalloc(c, &s.dW, 4 * (size_t)nc));
+  CK(dalloc(c, &s.dt, nc));
+  CK(dalloc(c, &s.W0, 4 * (size_t)nc));
+  CK(dalloc(c, &s.err, 2));
+  if (visc) {
+    CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
+    CK(dalloc(c, &s.lamv, nc));
+    CK(dalloc(c, &s.mu, nc));
+  }
+  if (sst) {
+    CK(dalloc(c, &s.tU, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tW, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tUn, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tWn, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tgrad, 4 * (size_t)nc));
+    CK(dalloc(c, &s.F1, nc));
+    CK(dalloc(c, &s.F2, nc));
+    CK(dalloc(c, &s.mut, nc));
+    CK(dalloc(c, &s.mutn, nc));
+    CK(dalloc(c, &s.tR, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tdW1, 2 * (size_t)nc));
+    CK(dalloc(c, &s.tdW, 2 * (size_t)nc));
+  }
+  CK(dalloc(c, &c->d_partial, c->nblk));
+  CK(dalloc(c, &c->d_force, 2));
+  CK(ensure_norms(c, 16));
+#undef CK
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    usg_ctx_destroy(c);
+    return fail(USG_ERR_HIP, "stream sync after upload");
+  }
+  *out = c;
+  return usg_init_freestream(c);
+}
+
+int usg_set_stream(usg_ctx *c, void *st) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  c->stream = st ? (hipStream_t)st : c->own;
+  return USG_OK;
+}
+
+int usg_sync(usg_ctx *c) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return USG_OK;
+}
+
+int usg_init_freestream(usg_ctx *c) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  const bool sst = c->prm.flow == USG_SST;
+  hipLaunchKernelGGL(k_freestream, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, (int)sst);
+  hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
+  if (sst) hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_residual = false;
+  return USG_OK;
+}
+
+int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
+  if (!c || !U) return fail(USG_ERR_ARG, "null argument");
+  const int nc = c->hm.nc;
+  std::vector<double> soa(4 * (size_t)nc);
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < 4; j++) soa[(size_t)j * nc + i] = U[4 * (size_t)i + j];
+  HIPCHK(hipMemcpyAsync(c->ds.U, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
+  if (c->prm.flow == USG_SST && turb) {
+    std::vector<double> t(2 * (size_t)nc);
+    for (int i = 0; i < nc; i++)
+      for (int j = 0; j < 2; j++) t[(size_t)j * nc + i] = turb[2 * (size_t)i + j];
+    HIPCHK(hipMemcpyAsync(c->ds.tU, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_residual = false;
+  return USG_OK;
+}
+
+int usg_residual(usg_ctx *c) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  launch_residual_any(c);
+  HIPCHK(hipGetLastError());
+  c->have_residual = true;
+  return check_err(c);
+}
+
+int usg_lusgs_step(usg_ctx *c, int n, double *res_hist) {
+  if (!c || n < 0) return fail(USG_ERR_ARG, "bad argument");
+  if (res_hist) {
+    int rc = ensure_norms(c, n);
+    if (rc) return rc;
+  }
+  for (int it = 0; it < n; it++) {
+    launch_residual_any(c);
+    if (res_hist) launch_norm(c, c->d_norms + it);
+    switch (c->prm.flow) {
+      case USG_EULER: launch_sweeps<USG_EULER>(c); break;
+      case USG_LAMINAR: launch_sweeps<USG_LAMINAR>(c); break;
+      default: launch_sweeps<USG_SST>(c); break;
+    }
+    swap_state(c);
+  }
+  HIPCHK(hipGetLastError());
+  if (res_hist && n > 0)
+    HIPCHK(hipMemcpyAsync(res_hist, c->d_norms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->have_residual = false;
+  return check_err(c);
+}
+
+int usg_rk5_step(usg_ctx *c, int n, double *res_hist) {
+  if (!c || n < 0) return fail(USG_ERR_ARG, "bad argument");
+  if (c->prm.flow == USG_SST) return fail(USG_ERR_ARG, "RK5 is provided for Euler and laminar flow");
+  if (res_hist) {
+    int rc = ensure_norms(c, n);
+    if (rc) return rc;
+  }
+  const double alpha[5] = {1.0 / 4, 1.0 / 6, 3.0 / 8, 1.0 / 2, 1.0};
+  for (int it = 0; it < n; it++) {
+    hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W0);
+    for (int st = 0; st < 5; st++) {
+      launch_residual_any(c);
+      if (res_hist && st == 0) launch_norm(c, c->d_norms + it);
+      hipLaunchKernelGGL(k_rk_stage, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->prm.cfl_rk,
+                         alpha[st], st, 0);
+    }
+  }
+  HIPCHK(hipGetLastError());
+  if (res_hist && n > 0)
+    HIPCHK(hipMemcpyAsync(res_hist, c->d_norms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  c->have_residual = false;
+  return check_err(c);
+}
+
+static int soa_to_aos(usg_ctx *c, const double *dev, int nv, double *out, long count, bool has) {
+  const int nc = c->hm.nc;
+  if (count != (long)nv * nc) return fail(USG_ERR_ARG, "count mismatch");
+  if (!has || !dev) return fail(USG_ERR_ARG, "field not available for this flow type");
+  std::vector<double> tmp((size_t)nv * nc);
+  HIPCHK(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < nv; j++) out[(size_t)i * nv + j] = tmp[(size_t)j * nc + i];
+  return USG_OK;
+}
+
+int usg_get_field(usg_ctx *c, int field, double *out, long count) {
+  if (!c || !out) return fail(USG_ERR_ARG, "null argument");
+  const DevState &s = c->ds;
+  const bool visc = c->prm.flow != USG_EULER, sst = c->prm.flow == USG_SST;
+  switch (field) {
+    case USG_F_PRIM: return soa_to_aos(c, s.U, 4, out, count, true);
+    case USG_F_CONS: return soa_to_aos(c, s.W, 4, out, count, true);
+    case USG_F_RES: return soa_to_aos(c, s.R, 4, out, count, true);
+    case USG_F_GRAD: return soa_to_aos(c, s.grad, 8, out, count, true);
+    case USG_F_LAMC: return soa_to_aos(c, s.lamc, 1, out, count, true);
+    case USG_F_PHI: return soa_to_aos(c, s.phi, 4, out, count, true);
+    case USG_F_DW1: return soa_to_aos(c, s.dW1, 4, out, count, true);
+    case USG_F_DW: return soa_to_aos(c, s.dW, 4, out, count, true);
+    case USG_F_LAMV: return soa_to_aos(c, s.lamv, 1, out, count, visc);
+    case USG_F_GRADT: return soa_to_aos(c, s.gradT, 2, out, count, visc);
+    case USG_F_TURB: return soa_to_aos(c, s.tU, 2, out, count, sst);
+    case USG_F_TRES: return soa_to_aos(c, s.tR, 2, out, count, sst);
+    case USG_F_MUT: return soa_to_aos(c, s.mut, 1, out, count, sst);
+    case USG_F_F1: return soa_to_aos(c, s.F1, 1, out, count, sst);
+    case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
+    case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
+    default: return fail(USG_ERR_ARG, "unknown field");
+  }
+}
+
+int usg_res_l2(usg_ctx *c, double *out) {
+  if (!c || !out) return fail(USG_ERR_ARG, "null argument");
+  int rc = ensure_norms(c, 1);
+  if (rc) return rc;
+  launch_norm(c, c->d_norms);
+  HIPCHK(hipMemcpyAsync(out, c->d_norms, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return USG_OK;
+}
+
+int usg_forces(usg_ctx *c, double *cl, double *cd) {
+  if (!c || !cl || !cd) return fail(USG_ERR_ARG, "null argument");
+  int nw = (int)c->hm.wall_faces.size();
+  hipLaunchKernelGGL(k_forces, dim3(1), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->d_wall_faces,
+                     c->d_wall_cells, nw, c->d_force);
+  double f[2];
+  HIPCHK(hipMemcpyAsync(f, c->d_force, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double q = 0.5 * 1.4 * c->prm.mach * c->prm.mach;
+  double ca = std::cos(c->fs_aoa_rad), sa = std::sin(c->fs_aoa_rad);
+  *cl = (-f[0] * sa + f[1] * ca) / q;
+  *cd = (f[0] * ca + f[1] * sa) / q;
+  return USG_OK;
+}
+
+int usg_sweep_waves(usg_ctx *c, int *nfwd, int *nbwd) {
+  if (!c || !nfwd || !nbwd) return fail(USG_ERR_ARG, "null argument");
+  *nfwd = (int)c->hm.fwd_off.size() - 1;
+  *nbwd = (int)c->hm.bwd_off.size() - 1;
+  return USG_OK;
+}
+
+int usg_get_geometry(usg_ctx *c, double *vol, double *cc, double *fn, double *fa, double *fc, double *wdist) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  const HostMesh &h = c->hm;
+  if (vol) std::memcpy(vol, h.vol.data(), h.vol.size() * sizeof(double));
+  if (cc) std::memcpy(cc, h.cc.data(), h.cc.size() * sizeof(double));
+  if (fn) std::memcpy(fn, h.fn.data(), h.fn.size() * sizeof(double));
+  if (fa) std::memcpy(fa, h.fa.data(), h.fa.size() * sizeof(double));
+  if (fc) std::memcpy(fc, h.fc.data(), h.fc.size() * sizeof(double));
+  if (wdist) std::memcpy(wdist, h.wdist.data(), h.wdist.size() * sizeof(double));
+  return USG_OK;
+}
+
+int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *fn, double *fa, double *fc,
+                   double *wdist, int *nfwd, int *nbwd) {
+  if (!mesh) return fail(USG_ERR_ARG, "null mesh");
+  HostMesh h;
+  std::string e = build_mesh(h, mesh->nn, mesh->xy, mesh->nf, mesh->fnode, mesh->flc, mesh->frc, mesh->nc);
+  if (!e.empty()) return fail(USG_ERR_MESH, e);
+  if (vol) std::memcpy(vol, h.vol.data(), h.vol.size() * sizeof(double));
+  if (cc) std::memcpy(cc, h.cc.data(), h.cc.size() * sizeof(double));
+  if (fn) std::memcpy(fn, h.fn.data(), h.fn.size() * sizeof(double));
+  if (fa) std::memcpy(fa, h.fa.data(), h.fa.size() * sizeof(double));
+  if (fc) std::memcpy(fc, h.fc.data(), h.fc.size() * sizeof(double));
+  if (wdist) std::memcpy(wdist, h.wdist.data(), h.wdist.size() * sizeof(double));
+  if (nfwd) *nfwd = (int)h.fwd_off.size() - 1;
+  if (nbwd) *nbwd = (int)h.bwd_off.size() - 1;
+  return USG_OK;
+}
+
+int usg_timing_enable(usg_ctx *c, int on) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  c->timing = on != 0;
+  c->ev_used = 0;
+  return USG_OK;
+}
+
+int usg_kernel_time(usg_ctx *c, double *total_ms, long *launches) {
+  if (!c || !total_ms || !launches) return fail(USG_ERR_ARG, "null argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double t = 0;
+  for (size_t i = 0; i < c->ev_used; i++) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+    t += ms;
+  }
+  *total_ms = t;
+  *launches = (long)c->ev_used;
+  c->ev_used = 0;
+  return USG_OK;
+}
+
+int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const int *nw, const int *donor,
+                      const double *w, void *st) {
+  if (!dev_q || nvar <= 0 || nrecv < 0 || (nrecv && (!recv || !nw || !donor || !w)))
+    return fail(USG_ERR_ARG, "bad argument");
+  if (nrecv == 0) return USG_OK;
+  hipStream_t stream = (hipStream_t)st;
+  hipLaunchKernelGGL(k_stencil, dim3(grid(nrecv * nvar)), dim3(kBlock), 0, stream, (double *)dev_q, nvar, nrecv,
+                     recv, nw, donor, w);
+  HIPCHK(hipGetLastError());
+  return USG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+"""Python front end of libusg.so (the C ABI in include/usg.h).
+
+Mirrors the reference's driver-facing interface: a :class:`Solver` is what
+``WrapSolver`` holds after ``setTopology``/``setParams``/``init``
+(UnstructSolver/SolverWrapper.h:22-68) -- one mesh, one flow strategy
+(``Euler2D`` / ``ViscousFlow2D``, laminar or SST) and its integrator (LU-SGS or
+RK5) -- running on the GPU.  Host arrays use the reference's AoS layout
+(``U[cell, var]`` with ``U = (rho, p, u, v)``).
+
+The library is loaded from the package directory; there is no CPU fallback:
+if ``libusg.so`` is missing or no GPU is visible, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import mesh as _mesh
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libusg.so")
+
+EULER, LAMINAR, SST = 0, 1, 2
+FP64, MIXED = 0, 1
+F_PRIM, F_CONS, F_RES, F_GRAD, F_LAMC, F_PHI, F_DW1, F_DW = range(8)
+F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW = range(8, 16)
+
+_FIELD_WIDTH = {F_PRIM: 4, F_CONS: 4, F_RES: 4, F_GRAD: 8, F_LAMC: 1, F_PHI: 4, F_DW1: 4, F_DW: 4,
+                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class UsgError(RuntimeError):
+    pass
+
+
+class _MeshDesc(C.Structure):
+    _fields_ = [("nn", C.c_int), ("nf", C.c_int), ("nc", C.c_int), ("xy", _dp), ("fnode", _ip),
+                ("flc", _ip), ("frc", _ip)]
+
+
+class _Params(C.Structure):
+    _fields_ = [("flow", C.c_int), ("precision", C.c_int), ("mach", C.c_double), ("aoa_deg", C.c_double),
+                ("reynolds", C.c_double), ("cfl_lusgs", C.c_double), ("cfl_rk", C.c_double)]
+
+
+EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "usg_init_freestream",
+           "usg_set_prim", "usg_residual", "usg_lusgs_step", "usg_rk5_step", "usg_get_field", "usg_res_l2",
+           "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
+           "usg_kernel_time", "usg_stencil_apply", "usg_last_error"]
+
+_lib = None
+
+
+def lib():
+    """Load libusg.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UsgError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.usg_ctx_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_Params), C.POINTER(vp)]
+        L.usg_ctx_destroy.argtypes = [vp]
+        L.usg_set_stream.argtypes = [vp, vp]
+        L.usg_sync.argtypes = [vp]
+        L.usg_init_freestream.argtypes = [vp]
+        L.usg_set_prim.argtypes = [vp, _dp, _dp]
+        L.usg_residual.argtypes = [vp]
+        L.usg_lusgs_step.argtypes = [vp, C.c_int, _dp]
+        L.usg_rk5_step.argtypes = [vp, C.c_int, _dp]
+        L.usg_get_field.argtypes = [vp, C.c_int, _dp, C.c_long]
+        L.usg_res_l2.argtypes = [vp, _dp]
+        L.usg_forces.argtypes = [vp, _dp, _dp]
+        L.usg_sweep_waves.argtypes = [vp, _ip, _ip]
+        L.usg_get_geometry.argtypes = [vp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.usg_mesh_build.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
+        L.usg_timing_enable.argtypes = [vp, C.c_int]
+        L.usg_kernel_time.argtypes = [vp, _dp, C.POINTER(C.c_long)]
+        L.usg_stencil_apply.argtypes = [vp, C.c_int, C.c_int, _ip, _ip, _ip, _dp, vp]
+        L.usg_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise UsgError(f"usg error {rc}: {lib().usg_last_error().decode()}")
+
+
+def _desc(m: _mesh.Mesh):
+    keep = [np.ascontiguousarray(m.xy, np.float64), np.ascontiguousarray(m.fnode, np.int32),
+            np.ascontiguousarray(m.flc, np.int32), np.ascontiguousarray(m.frc, np.int32)]
+    d = _MeshDesc(m.nn, m.nf, m.nc, keep[0].ctypes.data_as(_dp), keep[1].ctypes.data_as(_ip),
+                  keep[2].ctypes.data_as(_ip), keep[3].ctypes.data_as(_ip))
+    return d, keep
+
+
+def mesh_geometry(m: _mesh.Mesh) -> dict:
+    """Host geometry and wave counts of the product (no GPU needed)."""
+    d, keep = _desc(m)
+    out = {"vol": np.zeros(m.nc), "cc": np.zeros((m.nc, 2)), "fn": np.zeros((m.nf, 2)),
+           "fa": np.zeros(m.nf), "fc": np.zeros((m.nf, 2)), "wdist": np.zeros(m.nc)}
+    a, b = C.c_int(), C.c_int()
+    _check(lib().usg_mesh_build(C.byref(d), *(out[k].ctypes.data_as(_dp) for k in
+                                              ("vol", "cc", "fn", "fa", "fc", "wdist")), C.byref(a), C.byref(b)))
+    out["nfwd"], out["nbwd"] = a.value, b.value
+    return out
+
+
+class Solver:
+    """One mesh + flow strategy + integrator on the GPU (WrapSolver equivalent)."""
+
+    def __init__(self, m: _mesh.Mesh, flow: int = EULER, mach: float = 0.5, aoa: float = 0.0,
+                 reynolds: float = 1e6, precision: int = FP64, cfl_lusgs: float = 400.0, cfl_rk: float = 0.8):
+        self.mesh = m
+        self.nc = m.nc
+        self.flow = flow
+        d, keep = _desc(m)
+        p = _Params(flow, precision, mach, aoa, reynolds, cfl_lusgs, cfl_rk)
+        h = C.c_void_p()
+        _check(lib().usg_ctx_create(C.byref(d), C.byref(p), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().usg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- state -----------------------------------------------------------
+    def init_freestream(self):
+        _check(lib().usg_init_freestream(self._h))
+
+    def set_prim(self, U, turb=None):
+        U = np.ascontiguousarray(U, np.float64).reshape(self.nc, 4)
+        tp = None
+        if turb is not None:
+            turb = np.ascontiguousarray(turb, np.float64).reshape(self.nc, 2)
+            tp = turb.ctypes.data_as(_dp)
+        _check(lib().usg_set_prim(self._h, U.ctypes.data_as(_dp), tp))
+
+    def field(self, fid: int) -> np.ndarray:
+        w = _FIELD_WIDTH[fid]
+        out = np.zeros(self.nc * w)
+        _check(lib().usg_get_field(self._h, fid, out.ctypes.data_as(_dp), out.size))
+        return out.reshape(self.nc, w) if w > 1 else out
+
+    @property
+    def U(self):
+        return self.field(F_PRIM)
+
+    @property
+    def R(self):
+        return self.field(F_RES)
+
+    # --- work ------------------------------------------------------------
+    def residual(self):
+        _check(lib().usg_residual(self._h))
+
+    def lusgs(self, n: int = 1, history: bool = False):
+        h = np.zeros(max(n, 1)) if history else None
+        _check(lib().usg_lusgs_step(self._h, n, h.ctypes.data_as(_dp) if history else None))
+        return h[:n] if history else None
+
+    def rk5(self, n: int = 1, history: bool = False):
+        h = np.zeros(max(n, 1)) if history else None
+        _check(lib().usg_rk5_step(self._h, n, h.ctypes.data_as(_dp) if history else None))
+        return h[:n] if history else None
+
+    def res_l2(self) -> float:
+        v = C.c_double()
+        _check(lib().usg_res_l2(self._h, C.byref(v)))
+        return v.value
+
+    def forces(self):
+        cl, cd = C.c_double(), C.c_double()
+        _check(lib().usg_forces(self._h, C.byref(cl), C.byref(cd)))
+        return cl.value, cd.value
+
+    def waves(self):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().usg_sweep_waves(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def sync(self):
+        _check(lib().usg_sync(self._h))
+
+    def timing(self, on: bool):
+        _check(lib().usg_timing_enable(self._h, int(on)))
+
+    def kernel_time(self):
+        t, n = C.c_double(), C.c_long()
+        _check(lib().usg_kernel_time(self._h, C.byref(t), C.byref(n)))
+        return t.value, n.value
