@@ -41,6 +41,7 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
     """The oracle (C restatement of the reference's serial path, 1 core) on
     the same mesh and inputs, iterated until ``seconds`` of CPU work."""
     from oracle import oracle as O
+    print("# cpu baseline: building the oracle's geometry", file=sys.stderr, flush=True)
     om = O.OracleMesh(m)
     s = O.OracleSolver(om, flow, mach, aoa, re)
     n = 0
@@ -48,6 +49,7 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
     while True:
         s.lusgs_step()
         n += 1
+        print(f"# cpu baseline iteration {n}: {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
         if time.perf_counter() - t0 >= seconds:
             break
     dt = time.perf_counter() - t0

@@ -227,11 +227,16 @@ orc_mesh *orc_mesh_create(int nn, const double *xy, int nf, const int *fnode,
     m->fc[2 * k] = sx / 2;
     m->fc[2 * k + 1] = sy / 2;
   }
-  /* UnstructBlock2D::ComputeWallCellDistance (UnstructSolver/UnstructBlock.h:265-298) */
+  /* UnstructBlock2D::ComputeWallCellDistance (UnstructSolver/UnstructBlock.h:265-298):
+   * wall faces collected in ascending order (the reference's std::set). */
+  int nw = 0;
+  int *wf = (int *)xcalloc(nf, sizeof(int));
+  for (int k = 0; k < nf; k++)
+    if (frc[k] == ORC_WALL) wf[nw++] = k;
   for (int i = 0; i < nc; i++) {
     double d = 100000000000.0;
-    for (int k = 0; k < nf; k++) {
-      if (frc[k] != ORC_WALL) continue;
+    for (int w = 0; w < nw; w++) {
+      int k = wf[w];
       double cx = m->cc[2 * i] - m->fc[2 * k];
       double cy = m->cc[2 * i + 1] - m->fc[2 * k + 1];
       double dist = fabs(dot2(cx, cy, m->fn[2 * k], m->fn[2 * k + 1]));
@@ -239,6 +244,7 @@ orc_mesh *orc_mesh_create(int nn, const double *xy, int nf, const int *fnode,
     }
     m->wdist[i] = d;
   }
+  free(wf);
   return m;
 }
 
