@@ -85,6 +85,8 @@ def lib():
         L.orc_forces.argtypes = [C.POINTER(_Solver), _dp, _dp]
         L.orc_hllc_face.argtypes = [_dp, _dp, _dp, C.c_double, _dp, _ip]
         L.orc_lusgs_sweeps.argtypes = [C.POINTER(_Solver)]
+        L.orc_cell_volume.restype = C.c_double
+        L.orc_cell_volume.argtypes = [_dp, C.c_int]
         L.orc_prim_to_cons.argtypes = [_dp, _dp]
         L.orc_cons_to_prim.argtypes = [_dp, _dp]
         L.orc_lusgs_levels.argtypes = [C.POINTER(_Mesh), _ip, _ip, _ip, _ip]
@@ -212,3 +214,8 @@ def cons_to_prim(W):
     U = np.zeros(4)
     lib().orc_cons_to_prim(W.ctypes.data_as(_dp), U.ctypes.data_as(_dp))
     return U
+
+
+def cell_volume(xy):
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    return lib().orc_cell_volume(xy.ctypes.data_as(_dp), xy.shape[0])

@@ -144,6 +144,17 @@ static double cell_volume_2d(double xv[4][2], int nvert) {
   return area;
 }
 
+/* computeVolume(2, xv, nvert) for a standalone vertex list (known answers of
+ * UniTioga/mul_test.cpp:123-161). */
+double orc_cell_volume(const double *xy, int nv) {
+  double xv[4][2];
+  for (int v = 0; v < nv && v < 4; v++) {
+    xv[v][0] = xy[2 * v];
+    xv[v][1] = xy[2 * v + 1];
+  }
+  return cell_volume_2d(xv, nv);
+}
+
 orc_mesh *orc_mesh_create(int nn, const double *xy, int nf, const int *fnode,
                           const int *flc, const int *frc, int nc) {
   orc_mesh *m = (orc_mesh *)xcalloc(1, sizeof(orc_mesh));

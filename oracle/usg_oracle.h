@@ -89,6 +89,7 @@ double orc_dw_rho_sq(orc_solver *s);             /* sum dW_rho^2 (postprocessUpd
 void orc_forces(orc_solver *s, double *cl, double *cd);
 /* standalone pieces for invariant tests */
 void orc_lusgs_sweeps(orc_solver *s);
+double orc_cell_volume(const double *xy, int nv);
 void orc_prim_to_cons(const double *U, double *W);
 void orc_cons_to_prim(const double *W, double *U);
 void orc_hllc_face(const double *UL, const double *UR, const double *n, double area,

@@ -1,0 +1,293 @@
+"""Pins the CPU oracle (and the product's host geometry) without a GPU.
+
+Known answers from the reference's own test code, closed forms and
+invariants chosen so that a plausible slip -- a dropped term, a flipped sign,
+a swapped left/right cell, a wrong neighbour set in a sweep -- fails one of
+them.  Each test names what it would catch.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tmp_amd import mesh
+
+from .common import REFERENCE, bump, part2, perturbed
+
+G = 1.4
+
+
+# --------------------------------------------------------------------------
+# geometry
+# --------------------------------------------------------------------------
+
+def test_volume_known_answers():
+    """UniTioga/mul_test.cpp:123-161: triangle (0,1),(1,1),(1,0) has area
+    0.5; trapezoid ABCD (0,0),(1,0),(2,3),(0,3) has area 4.5 in the vertex
+    orders ABCD, ACBD and ADCB (catches a wrong diagonal choice)."""
+    assert O.cell_volume([[0, 1], [1, 1], [1, 0]]) == 0.5
+    A, B, Cc, D = [0, 0], [1, 0], [2, 3], [0, 3]
+    for order in ([A, B, Cc, D], [A, Cc, B, D], [A, D, Cc, B]):
+        assert O.cell_volume(order) == 4.5
+
+
+def test_geometry_closure_and_area():
+    """Sum of n*A over each cell's faces (outward) vanishes; cell areas add up
+    to the domain area (catches a swapped normal or a wrong cell vertex order)."""
+    m = mesh.rect(7, 5, lx=2.0, ly=1.0, jitter=0.2, seed=3)
+    om = O.OracleMesh(m)
+    acc = np.zeros((m.nc, 2))
+    np.add.at(acc, m.flc, om.fn * om.fa[:, None])
+    inner = m.frc >= 0
+    np.add.at(acc, m.frc[inner], -(om.fn * om.fa[:, None])[inner])
+    assert np.abs(acc).max() < 1e-14
+    assert abs(om.vol.sum() - 2.0) < 1e-14
+    t = mesh.rect(6, 4, jitter=0.2, seed=4, triangles=True)
+    assert abs(O.OracleMesh(t).vol.sum() - 1.0) < 1e-14
+
+
+def test_wall_distance_flat_channel():
+    """Projected wall distance (UnstructBlock.h:281-289) in a flat channel is
+    min(y_c, 1 - y_c) for every cell centre."""
+    m = mesh.channel_bump(ni=30, nj=12, nq=3, seed=5, height=0.0)
+    om = O.OracleMesh(m)
+    y = om.cc[:, 1]
+    assert np.allclose(om.wdist, np.minimum(y, 1 - y), rtol=0, atol=1e-15)
+
+
+def test_product_geometry_bit_exact():
+    """Product host geometry (tmp_amd/csrc/geometry.cpp) == oracle geometry,
+    bit for bit, and the same LU-SGS wave counts."""
+    from tmp_amd import usg
+    meshes = [part2(), bump(), mesh.rect(5, 4, jitter=0.1, seed=1, triangles=True)]
+    for m in meshes:
+        g = usg.mesh_geometry(m)
+        om = O.OracleMesh(m)
+        for k in ("vol", "cc", "fn", "fa", "fc", "wdist"):
+            assert np.array_equal(g[k], getattr(om, k)), (m.name, k)
+        fl, bl, nfl, nbl = om.levels()
+        assert (g["nfwd"], g["nbwd"]) == (nfl, nbl)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE, "mix_element_laminar.grd")),
+                    reason="reference fixtures not mounted")
+def test_reference_fixtures_shape_and_levels():
+    """Counts of the shipped meshes and their LU-SGS DAG depth in file
+    numbering: 529 levels for mix_element_laminar.grd, 122 for part2.grd."""
+    mx = mesh.read_grd(os.path.join(REFERENCE, "mix_element_laminar.grd"), -1, -2)
+    assert (mx.nn, mx.nf, mx.nc) == (40359, 80308, 39949)
+    assert (mx.frc == mesh.WALL).sum() == 199 and (mx.frc == mesh.FARFIELD).sum() == 621
+    assert O.OracleMesh(mx).levels()[2] == 529
+    p2 = part2()
+    assert (p2.nn, p2.nf, p2.nc) == (2323, 4545, 2222)
+    assert O.OracleMesh(p2).levels()[2] == 122
+
+
+def test_grd_roundtrip_and_bad_tag(tmp_path):
+    m = bump()
+    f = tmp_path / "b.grd"
+    mesh.write_grd(str(f), m)
+    r = mesh.read_grd(str(f), -1, -2)
+    assert np.array_equal(r.xy, m.xy) and np.array_equal(r.fnode, m.fnode)
+    assert np.array_equal(r.flc, m.flc) and np.array_equal(r.frc, m.frc)
+    with pytest.raises(ValueError):
+        mesh.read_grd(str(f), -1, -7)
+
+
+# --------------------------------------------------------------------------
+# state conversions and free stream
+# --------------------------------------------------------------------------
+
+def test_prim_cons_known_answer():
+    """VariableTest.cpp:7-24: (rho, rhoE, rhou, rhov) = (0.8, 3.6113,
+    0.298476, 0.898283) -> primitive -> conservative round trip; pressure
+    closed form p = 0.4 (rhoE - |m|^2 / 2 rho)."""
+    W = np.array([0.8, 3.6113, 0.298476, 0.898283])
+    U = O.cons_to_prim(W)
+    p = 0.4 * (W[1] - (W[2] ** 2 + W[3] ** 2) / (2 * W[0]))
+    assert abs(U[1] - p) < 1e-15
+    assert U[2] == W[2] / W[0] and U[3] == W[3] / W[0]
+    W2 = O.prim_to_cons(U)
+    assert np.allclose(W2, W, rtol=4e-16, atol=0)
+
+
+@pytest.mark.parametrize("mach,aoa", [(0.5, 0.0), (0.15, 4.0), (0.85, -3.0)])
+def test_freestream_closed_form(mach, aoa):
+    """set_fs_variable2 + update_nondim_variable: U_inf = (1, 1,
+    M sqrt(g) cos a, M sqrt(g) sin a) with a = 3.141593 * AoA / 180."""
+    om = O.OracleMesh(part2())
+    s = O.OracleSolver(om, O.EULER, mach, aoa, 1e6)
+    a = 3.141593 * aoa / 180
+    fs = list(s.s.fs_primVar)[:4]
+    assert fs[0] == 1.0 and fs[1] == 1.0
+    assert abs(fs[2] - mach * np.sqrt(G) * np.cos(a)) < 4e-16
+    assert abs(fs[3] - mach * np.sqrt(G) * np.sin(a)) < 4e-16
+
+
+def test_sutherland_at_freestream():
+    """computeMu at T = T_inf gives mu = sqrt(gamma) M / Re."""
+    m = mesh.rect(4, 3)
+    s = O.OracleSolver(O.OracleMesh(m), O.LAMINAR, 0.3, 0.0, 2500.0)
+    s.residual()
+    mu = s.field("mu", 1)
+    assert np.allclose(mu, np.sqrt(G) * 0.3 / 2500.0, rtol=2e-16, atol=0)
+
+
+# --------------------------------------------------------------------------
+# residual invariants
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("flow", [O.EULER, O.LAMINAR])
+def test_freestream_preservation(flow):
+    """A uniform free stream is steady: on a far-field box every residual is
+    roundoff, the gradient vanishes and the limiter stays 1 (catches a wrong
+    Riemann-invariant branch or a non-closing flux sum)."""
+    m = mesh.rect(6, 5, jitter=0.15, seed=2)
+    s = O.OracleSolver(O.OracleMesh(m), flow, 0.6, 7.0, 1e5)
+    s.residual()
+    assert np.abs(s.R).max() < 1e-13
+    assert np.abs(s.field("gradU", 8)).max() < 1e-12
+    assert np.abs(s.field("phi", 4) - 1).max() < 1e-12
+
+
+def _euler_flux(U, n):
+    rho, p, u, v = U
+    vn = u * n[0] + v * n[1]
+    E = p / (G - 1) + 0.5 * rho * (u * u + v * v)
+    return np.array([rho * vn, (E + p) * vn, rho * u * vn + p * n[0], rho * v * vn + p * n[1]])
+
+
+def test_residual_telescoping():
+    """Interior fluxes cancel: sum over cells of R equals the sum of the
+    boundary face fluxes -- (0,0,p n A) at slip walls (HLLCFluxStrategy.cpp
+    :126-135) and the Euler flux of the face state at far-field faces
+    (catches a sign error on the right cell or a lost face)."""
+    m = mesh.channel_bump(ni=24, nj=10, nq=2, seed=7)
+    om = O.OracleMesh(m)
+    s = O.OracleSolver(om, O.EULER, 0.5, 0.0, 1e6)
+    s.set_prim(perturbed(m, om.cc, list(s.s.fs_primVar)[:4], amp=0.03))
+    s.residual()
+    Ue = s.field("U_edge", 4, "nf")
+    tot = np.zeros(4)
+    for f in np.nonzero(m.frc < 0)[0]:
+        n, A = om.fn[f], om.fa[f]
+        if m.frc[f] == mesh.WALL:
+            tot += np.array([0, 0, Ue[f, 1] * n[0], Ue[f, 1] * n[1]]) * A
+        else:
+            tot += _euler_flux(Ue[f], n) * A
+    assert np.allclose(s.R.sum(0), tot, rtol=1e-11, atol=1e-13)
+
+
+def test_limiter_bounds():
+    """Venkatakrishnan phi is in (0, 1] (initial 1, min over faces)."""
+    m = bump()
+    om = O.OracleMesh(m)
+    s = O.OracleSolver(om, O.EULER, 0.5, 0.0, 1e6)
+    s.set_prim(perturbed(m, om.cc, list(s.s.fs_primVar)[:4], amp=0.2, seed=3))
+    s.residual()
+    phi = s.field("phi", 4)
+    assert phi.min() > 0 and phi.max() <= 1.0
+    assert (phi < 1).mean() > 0.1  # the limiter is active somewhere
+
+
+def _rand_state(rng, sup=False):
+    rho = rng.uniform(0.5, 2.0)
+    p = rng.uniform(0.5, 2.0)
+    c = np.sqrt(G * p / rho)
+    m = rng.uniform(1.2, 2.5) if sup else rng.uniform(0.0, 0.9)
+    th = rng.uniform(0, 2 * np.pi)
+    return np.array([rho, p, m * c * np.cos(th), m * c * np.sin(th)])
+
+
+def test_hllc_consistency_and_symmetry():
+    """HLLC: F(U,U) equals the physical flux; F(UL,UR,n) = -F(UR,UL,-n)
+    (conservation); the p* star-pressure check holds (HLLCFluxStrategy.cpp
+    :77-86)."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        th = rng.uniform(0, 2 * np.pi)
+        n = np.array([np.cos(th), np.sin(th)])
+        A = rng.uniform(0.1, 2.0)
+        U = _rand_state(rng, sup=rng.uniform() < 0.3)
+        f, ok = O.hllc_face(U, U, n, A)
+        assert ok
+        assert np.allclose(f, _euler_flux(U, n) * A, rtol=1e-13, atol=1e-13)
+        UL, UR = _rand_state(rng), _rand_state(rng)
+        f1, ok1 = O.hllc_face(UL, UR, n, A)
+        f2, ok2 = O.hllc_face(UR, UL, -n, A)
+        assert ok1 and ok2
+        assert np.allclose(f1, -f2, rtol=1e-12, atol=1e-13)
+
+
+def test_hllc_supersonic_upwind():
+    """When both sides move supersonically along n the flux is the upwind
+    side's physical flux."""
+    rng = np.random.default_rng(12)
+    n = np.array([1.0, 0.0])
+    for _ in range(50):
+        UL = np.array([1.0, 1.0, 3.0 * np.sqrt(G), rng.uniform(-0.5, 0.5)])
+        UR = np.array([rng.uniform(0.8, 1.2), rng.uniform(0.8, 1.2), 3.0 * np.sqrt(G), 0.1])
+        f, ok = O.hllc_face(UL, UR, n, 1.0)
+        assert np.allclose(f, _euler_flux(UL, n), rtol=1e-14)
+        f, ok = O.hllc_face(UR * [1, 1, -1, -1], UL * [1, 1, -1, -1], n, 1.0)
+        assert np.allclose(f, _euler_flux(UL * [1, 1, -1, -1], n), rtol=1e-14)
+
+
+# --------------------------------------------------------------------------
+# LU-SGS against a dense solve (brute force)
+# --------------------------------------------------------------------------
+
+def _flux_jacobian(W, n, h=1e-7):
+    def F(w):
+        rho, rE, ru, rv = w
+        u, v = ru / rho, rv / rho
+        p = (G - 1) * (rE - 0.5 * rho * (u * u + v * v))
+        vn = u * n[0] + v * n[1]
+        return np.array([rho * vn, (rE + p) * vn, ru * vn + p * n[0], rv * vn + p * n[1]])
+    J = np.zeros((4, 4))
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h * max(1.0, abs(W[k]))
+        J[:, k] = (F(W + e) - F(W - e)) / (2 * e[k])
+    return J
+
+
+@pytest.mark.parametrize("triangles", [False, True])
+def test_lusgs_matches_dense_factorisation(triangles):
+    """For a tiny residual the two sweeps solve (D+L) D^-1 (D+U) dW = -R
+    with L/U the lower/upper neighbour blocks 0.5 (J_j(n_ij) - r_A I) A and
+    D = V/dt + 0.75 lambda_c (TopologyHolderStrategy.cpp:190-323).  Built
+    here from finite-difference flux Jacobians and solved densely."""
+    m = mesh.rect(4, 3, jitter=0.15, seed=9, triangles=triangles)
+    om = O.OracleMesh(m)
+    s = O.OracleSolver(om, O.EULER, 0.4, 5.0, 1e6)
+    U = perturbed(m, om.cc, list(s.s.fs_primVar)[:4], amp=0.05, seed=5)
+    s.set_prim(U)
+    s.residual()
+    rng = np.random.default_rng(1)
+    Rt = rng.normal(size=(m.nc, 4)) * 1e-9
+    s.R[:] = Rt
+    s.sweeps()
+    dW = s.field("dW", 4).copy()
+    W = s.field("W", 4)
+    lamc = s.field("lamc", 1)
+    nc = m.nc
+    D = om.vol / (400 * om.vol / lamc) + 0.75 * lamc
+    Lm = np.zeros((4 * nc, 4 * nc))
+    Um = np.zeros((4 * nc, 4 * nc))
+    for f in range(m.nf):
+        l, r = int(m.flc[f]), int(m.frc[f])
+        if r < 0:
+            continue
+        n, A = om.fn[f], om.fa[f]
+        for i, j, sgn in ((l, r, 1.0), (r, l, -1.0)):
+            Uj = U[j]
+            vn = Uj[2] * n[0] + Uj[3] * n[1]
+            rA = 1.5 * (abs(vn) + np.sqrt(G * Uj[1] / Uj[0]))
+            blk = 0.5 * (sgn * _flux_jacobian(W[j], n) - rA * np.eye(4)) * A
+            (Lm if j < i else Um)[4 * i:4 * i + 4, 4 * j:4 * j + 4] += blk
+    Dm = np.diag(np.repeat(D, 4))
+    M = (Dm + Lm) @ np.linalg.inv(Dm) @ (Dm + Um)
+    ref = np.linalg.solve(M, -Rt.ravel()).reshape(nc, 4)
+    assert np.allclose(dW, ref, rtol=2e-5, atol=1e-6 * np.abs(ref).max())

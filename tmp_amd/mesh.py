@@ -331,20 +331,20 @@ def _assemble(xy, quads, tris):
 
 
 def channel_bump(ni: int = 160, nj: int = 70, nq: int = 8, seed: int = 1001,
-                 jitter: float = 0.1) -> Mesh:
+                 jitter: float = 0.1, height: float = 0.1) -> Mesh:
     """Channel x in [-1,2], y in [0,1] with a 10 %-height circular-arc bump of
     chord 1 on y=0 (x in [0,1]).  Quads in the ``nq`` layers next to each
     wall, Delaunay-split triangles elsewhere; interior nodes jittered by
     U(-jitter, jitter) x local spacing (seeded).  Walls at y=0 and y=1,
     far field at x=-1 and x=2.  Default size ~19.8k cells."""
     rng = np.random.default_rng(seed)
-    h = 0.1
-    R = (0.25 + h * h) / (2 * h)
+    h = height
+    R = (0.25 + h * h) / (2 * h) if h > 0 else 1.0
     xs = np.linspace(-1.0, 2.0, ni + 1)
     eta = np.linspace(0.0, 1.0, nj + 1)
     # cluster toward both walls
     eta = 0.5 * (1 - np.cos(np.pi * eta)) * 0.6 + 0.4 * eta
-    ylo = np.where((xs > 0) & (xs < 1), np.sqrt(np.maximum(R * R - (xs - 0.5) ** 2, 0)) - (R - h), 0.0)
+    ylo = np.where((xs > 0) & (xs < 1) & (h > 0), np.sqrt(np.maximum(R * R - (xs - 0.5) ** 2, 0)) - (R - h), 0.0)
     X = np.repeat(xs[:, None], nj + 1, axis=1)
     Y = ylo[:, None] + (1.0 - ylo[:, None]) * eta[None, :]
     dx = np.gradient(xs)[:, None] * np.ones_like(X)
@@ -505,3 +505,37 @@ def airfoil_1m() -> Mesh:
     (y+ ~ 1 at Re 6e6), quad wall layers, isotropic triangles out to a far
     field at 50 chords, 810 points around the contour."""
     return airfoil_hybrid(ni=810, t=0.12)
+
+
+def rect(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0, bottom: int = FARFIELD, top: int = FARFIELD,
+         sides: int = FARFIELD, jitter: float = 0.0, seed: int = 0, triangles: bool = False) -> Mesh:
+    """Rectangle [0,lx]x[0,ly] of nx*ny quads (or Delaunay-split triangles),
+    interior nodes jittered by U(-jitter, jitter) x spacing."""
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(np.linspace(0, lx, nx + 1), np.linspace(0, ly, ny + 1), indexing="ij")
+    if jitter > 0:
+        jx = rng.uniform(-jitter, jitter, X.shape) * lx / nx
+        jy = rng.uniform(-jitter, jitter, X.shape) * ly / ny
+        inner = np.zeros_like(X, dtype=bool)
+        inner[1:-1, 1:-1] = True
+        X = X + np.where(inner, jx, 0)
+        Y = Y + np.where(inner, jy, 0)
+    xy = np.column_stack([X.ravel(), Y.ravel()])
+    nid = np.arange((nx + 1) * (ny + 1)).reshape(nx + 1, ny + 1)
+    q = np.stack([nid[:-1, :-1], nid[1:, :-1], nid[1:, 1:], nid[:-1, 1:]], axis=-1).reshape(-1, 4)
+    if triangles:
+        cells, nv = _assemble(xy, np.zeros((0, 4), np.int64), _split_quads(xy, q))
+    else:
+        cells, nv = _assemble(xy, q, np.zeros((0, 3), np.int64))
+    eps = 1e-9 * max(lx, ly)
+
+    def bc(x0, y0, x1, y1):
+        my = 0.5 * (y0 + y1)
+        out = np.full(x0.shape, sides)
+        out = np.where(my < eps, bottom, out)
+        out = np.where(my > ly - eps, top, out)
+        return out
+
+    m = faces_from_cells(xy, cells, nv, bc)
+    m.name = f"rect_{nx}x{ny}{'_tri' if triangles else ''}"
+    return m
