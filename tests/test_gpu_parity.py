@@ -133,3 +133,25 @@ def test_forces_match_oracle():
     clg, cdg = gpu.forces()
     assert abs(clg - clo) <= 1e-12 * max(1.0, abs(clo))
     assert abs(cdg - cdo) <= 1e-12 * max(1.0, abs(cdo))
+
+
+def test_cpp_adapter_driver():
+    """The C++ adapters (tmp_amd/host/usg_strategy.h: UnstructReader,
+    WrapSolver, GpuLUSGSIntegrator) driven like the reference's serial
+    scenario reproduce the oracle."""
+    import os
+    import subprocess
+    from .common import GOLDEN
+    exe = os.path.join(os.path.dirname(usg.LIB_PATH), "usg_run")
+    out = subprocess.run([exe, os.path.join(GOLDEN, "part2.grd"), "-2", "-2147483647", "Euler2D", "LUSGS",
+                          "0.5", "1e6", "2.0", "10", "1e-30"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    line = [l for l in out.stdout.splitlines() if l.startswith("iterations")][0]
+    tok = line.split()
+    res = float(tok[3])
+    u0 = [float(x.strip("(),")) for x in line.split("=")[1].split()]
+    orc = O.OracleSolver(O.OracleMesh(part2()), O.EULER, 0.5, 2.0, 1e6)
+    for _ in range(10):
+        orc.lusgs_step()
+    assert abs(res - orc.res_l2()) <= 1e-12 * orc.res_l2()
+    assert np.allclose(u0, orc.U[0], rtol=1e-12, atol=0)

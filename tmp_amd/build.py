@@ -17,6 +17,9 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result", "-Wno-unused-value"]
 
 
+DRIVER = os.path.join(HERE, "usg_run")
+
+
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     cmd = [hipcc, *FLAGS, *SOURCES, "-o", LIB + ".tmp"]
@@ -26,6 +29,12 @@ def build(verbose: bool = False) -> str:
     if verbose:
         print(r.stderr)
     os.replace(LIB + ".tmp", LIB)
+    # C++ driver over the host adapters (host-only, g++), linked to libusg.so
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "host", "usg_run.cpp"), "-o", DRIVER,
+           "-L" + HERE, "-lusg", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + r.stdout + r.stderr)
     return LIB
 
 

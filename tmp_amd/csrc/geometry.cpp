@@ -134,6 +134,19 @@ void waves(HostMesh &m) {
     std::vector<int> pos(m.bwd_off.begin(), m.bwd_off.end() - 1);
     for (int k = 0; k < nc; k++) m.bwd_cells[pos[lev[k]]++] = k;
   }
+  m.ranges.assign(1, 0);
+  for (int k = 0; k < nc; k++) {
+    int s0 = m.ranges.back();
+    for (int f = 0; f < m.cnf[k]; f++) {
+      int o = other(k, m.cface[4 * k + f]);
+      if (o >= s0 && o < k) {
+        m.ranges.push_back(k);
+        break;
+      }
+    }
+  }
+  m.ranges.push_back(nc);
+  m.use_ranges = m.ranges.size() - 1 == m.fwd_off.size() - 1;
 }
 
 }  // namespace

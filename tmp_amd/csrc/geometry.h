@@ -26,6 +26,12 @@ struct HostMesh {
   // LU-SGS waves: cells grouped by dependency level (ascending cell id inside)
   std::vector<int> fwd_cells, fwd_off;
   std::vector<int> bwd_cells, bwd_off;
+  // maximal contiguous ranges of mutually non-adjacent cells; when there are
+  // as many as forward levels (colour-blocked numbering) they are used as the
+  // waves of both sweeps (forward in order, backward in reverse: a cell's
+  // upper neighbours always lie in later ranges)
+  std::vector<int> ranges;
+  bool use_ranges = false;
   std::vector<int> wall_faces;
 };
 

@@ -20,6 +20,7 @@ struct DevMesh {
   int nc, nf;
   const double *vol, *ccx, *ccy, *wdist;
   const int *cf, *cnb;
+  const int *flc, *frc;  // face left cell, right cell or kWallD / kFarD
   const double *fnx, *fny, *fA, *fcx, *fcy;
 };
 
@@ -34,8 +35,13 @@ struct DevState {
   double *phi;          // [4][nc]
   double *R;            // [4][nc]
   double *dW1, *dW;     // [4][nc]
-  double *dt;           // [nc]
+  double *dt;           // [nc] LU-SGS local time step
+  double *dt_rk;        // [nc] RK5 time step, frozen at stage 0
   double *W0;           // [4][nc] RK5 stage origin
+  double *Fc, *Fv, *Ft; // face fluxes: convective [4][nf], viscous [4][nf], SST [2][nf]
+  double *D;            // [nc] LU-SGS diagonal
+  double *tD;           // [2][nc] SST diagonal
+  double *mprc;         // [nc] mu/Pr of the cell (laminar + turbulent) for r_A
   // SST
   double *tU, *tW, *tUn, *tWn;  // [2][nc]
   double *tgrad;                // [4][nc]: (2q+d)*nc

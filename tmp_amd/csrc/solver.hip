@@ -6,11 +6,14 @@
 // (faces visited k = 0..nf-1, each adding to its left and right cell) without
 // floating-point atomics.  Kernels per LU-SGS iteration:
 //   k_grad    boundary states, Green-Gauss gradients (+grad T, +grad k/omega),
-//             spectral radii, Venkatakrishnan limiter, SST F1/F2
-//   k_flux    limited reconstruction + HLLC, viscous and SST fluxes, residual
+//             spectral radii, Venkatakrishnan limiter, SST F1/F2   (per cell)
+//   k_face    limited reconstruction + HLLC, viscous and SST fluxes (per face,
+//             each face once)
+//   k_cell    residual = ordered gather of the face fluxes, SST source, and
+//             the LU-SGS diagonal factors                          (per cell)
 //   k_sweep   one launch per wave of the forward and the backward sweep; the
-//             last backward wave of each cell also writes the updated state
-//             into the second buffer (no separate update pass)
+//             backward wave of each cell also writes the updated state into
+//             the second buffer (no separate update pass)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -256,248 +259,206 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
 }
 
 // ---------------------------------------------------------------------------
-// k_flux: residual of one cell.  HLLCFluxStrategy::computeFlux
-// (HLLCFluxStrategy.cpp:4-171), then the second face pass of
-// LaminarFluxStrategy::computeFlux (LaminarFluxStrategy.h:110-216), then the
-// SST residual (SSTKWStrategy.h:783-1013, source sign as deviation D1 of the
-// oracle: -Q*V).
+// k_face: every flux of one face, computed once.  HLLCFluxStrategy::computeFlux
+// (HLLCFluxStrategy.cpp:4-171) with the limited reconstruction of
+// updateLeftRight (HLLCFluxStrategy.h:35-100); the viscous face flux of
+// LaminarFluxStrategy::computeFlux (LaminarFluxStrategy.h:110-216); the SST
+// convective minus diffusive face flux (SSTKWStrategy.h:616-659,783-920).
+// Outputs per face: Fc[4], Fv[4] (viscous), Ft[2] (SST), all area-weighted,
+// oriented out of the left cell.
 // ---------------------------------------------------------------------------
 template <int FLOW>
-__global__ __launch_bounds__(kBlock) void k_flux(const DevMesh m, const DevState s, const Consts k) {
+__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k) {
   using F = Flow<FLOW>;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nc = m.nc;
-  if (c >= nc) return;
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc, nf = m.nf;
+  if (f >= nf) return;
   const double Gamma = USG_GAMMA;
-  double Uc[4];
-  load4(s.U, nc, c, Uc);
-  double Rc[4] = {0, 0, 0, 0};
-  int bad = 0;
-  for (int sl = 0; sl < 4; sl++) {
-    Slot o;
-    if (!get_slot(m, sl, c, o)) break;
-    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
-    double Fx[4];
-    if (o.nb >= 0) {
-      double Unb[4], Ua[4], Ub[4];
-      load4(s.U, nc, o.nb, Unb);
-      if (o.left) {
-        reconstruct(m, s, c, o.f, Uc, Ua);
-        reconstruct(m, s, o.nb, o.f, Unb, Ub);
-      } else {
-        reconstruct(m, s, o.nb, o.f, Unb, Ua);
-        reconstruct(m, s, c, o.f, Uc, Ub);
-      }
-      if (!hllc(Ua, Ub, n0, n1, A, Fx)) bad = 1;
+  const int L = m.flc[f], Rr = m.frc[f];
+  const double n0 = m.fnx[f], n1 = m.fny[f], A = m.fA[f];
+  double UL[4], UR[4], Ue[4], Fx[4];
+  load4(s.U, nc, L, UL);
+  const bool inner = Rr >= 0;
+  if (inner) {
+    load4(s.U, nc, Rr, UR);
+    double Ua[4], Ub[4];
+    reconstruct(m, s, L, f, UL, Ua);
+    reconstruct(m, s, Rr, f, UR, Ub);
+    if (!hllc(Ua, Ub, n0, n1, A, Fx)) atomicAdd(&s.err[0], 1);
+  } else {
+    boundary_state<F::visc>(k, Rr, UL, n0, n1, Ue);
+    boundary_flux(Rr, Ue, n0, n1, A, Fx);
+  }
+  for (int j = 0; j < 4; j++) s.Fc[j * nf + f] = Fx[j];
+  if (!F::visc) return;
+  if (inner)
+    for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
+  double mue = sutherland(k, Ue[1] / Ue[0]);
+  double mutf = 0.0;
+  if (F::sst) {
+    if (inner) {
+      mutf = 0.5 * (s.mut[L] + s.mut[Rr]);
+    } else if (Rr == kWallD) {
+      mutf = 0.0;
     } else {
-      double Ue[4];
-      boundary_state<F::visc>(k, o.nb, Uc, n0, n1, Ue);
-      boundary_flux(o.nb, Ue, n0, n1, A, Fx);
-    }
-    if (o.left) {
-      for (int j = 0; j < 4; j++) Rc[j] += Fx[j];
-    } else {
-      for (int j = 0; j < 4; j++) Rc[j] -= Fx[j];
+      double Vn = dot2(Ue[2], Ue[3], n0, n1);
+      mutf = Vn > 0.0 ? s.mut[L] : k.fsnd_mut;
     }
   }
-  if (bad) atomicAdd(&s.err[0], 1);
-
-  if (F::visc) {
-    const double xc = m.ccx[c], yc = m.ccy[c];
-    const double mutc = F::sst ? s.mut[c] : 0.0;
-    double trc[2] = {0, 0};
-    double tUc[2] = {0, 0}, tgc[4] = {0, 0, 0, 0}, F1c = 0;
-    if (F::sst) {
-      tUc[0] = s.tU[c];
-      tUc[1] = s.tU[nc + c];
-      for (int q = 0; q < 4; q++) tgc[q] = s.tgrad[q * nc + c];
-      F1c = s.F1[c];
-      // SolveTotalTFlux source part (SSTKWStrategy.h:975-995)
-      double gux = s.grad[4 * nc + c], guy = s.grad[5 * nc + c];
-      double gvx = s.grad[6 * nc + c], gvy = s.grad[7 * nc + c];
-      double rho = Uc[0];
-      double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
-      double ddls1 = (guy + gvx) * (guy + gvx);
-      double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
-      double P = (ddls0 + ddls1) * mutc - ddls2;
-      double rdiv = dot2(tgc[0], tgc[1], tgc[2], tgc[3]);
-      double Cw = F1c * SST_CW1 + (1.0 - F1c) * SST_CW2;
-      double beta = F1c * SST_BETA1 + (1.0 - F1c) * SST_BETA2;
-      double kk = tUc[0], w = tUc[1];
-      double Q0 = P - SST_BSTAR * rho * kk * w;
-      double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
-      trc[0] = -(Q0 * m.vol[c]);
-      trc[1] = -(Q1 * m.vol[c]);
+  double mu_at_face = F::sst ? mue + mutf : mue;
+  double mpr = mue / k.pr;
+  if (F::sst) mpr = mpr + mutf / k.prt;
+  double efficient = 2.0 / 3.0 * mu_at_face;
+  double gE[3][2];
+  double l2r0 = 0, l2r1 = 0, l2sq = 0;
+  if (inner) {
+    l2r0 = m.ccx[Rr] - m.ccx[L];
+    l2r1 = m.ccy[Rr] - m.ccy[L];
+    l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
+    for (int j = 0; j < 2; j++) {
+      double a0 = (s.grad[(4 + 2 * j) * nc + L] + s.grad[(4 + 2 * j) * nc + Rr]) * 0.5;
+      double a1 = (s.grad[(5 + 2 * j) * nc + L] + s.grad[(5 + 2 * j) * nc + Rr]) * 0.5;
+      double sc = (UR[2 + j] - UL[2 + j] - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+      gE[j][0] = a0 + l2r0 * sc;
+      gE[j][1] = a1 + l2r1 * sc;
     }
-    for (int sl = 0; sl < 4; sl++) {
-      Slot o;
-      if (!get_slot(m, sl, c, o)) break;
-      const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
-      double Ue[4];
-      double Unb[4];
-      const int L = o.left ? c : o.nb, Rr = o.left ? o.nb : c;
-      if (o.nb >= 0) {
-        load4(s.U, nc, o.nb, Unb);
-        for (int j = 0; j < 4; j++) Ue[j] = o.left ? (Uc[j] + Unb[j]) / 2 : (Unb[j] + Uc[j]) / 2;
-      } else {
-        boundary_state<true>(k, o.nb, Uc, n0, n1, Ue);
-      }
-      double mue = sutherland(k, Ue[1] / Ue[0]);
-      double mutf = F::sst ? face_mut(k, s, o.nb, o.left, mutc, Ue, n0, n1) : 0.0;
-      double mu_at_face = F::sst ? mue + mutf : mue;
-      double mpr = mue / k.pr;
-      if (F::sst) mpr = mpr + mutf / k.prt;
-      double efficient = 2.0 / 3.0 * mu_at_face;
-      double gE[3][2];
-      if (o.nb >= 0) {
-        const double *UL = o.left ? Uc : Unb;
-        const double *UR = o.left ? Unb : Uc;
-        double l2r0 = m.ccx[Rr] - m.ccx[L], l2r1 = m.ccy[Rr] - m.ccy[L];
-        double l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
-        for (int j = 0; j < 2; j++) {
-          double a0 = (s.grad[(4 + 2 * j) * nc + L] + s.grad[(4 + 2 * j) * nc + Rr]) * 0.5;
-          double a1 = (s.grad[(5 + 2 * j) * nc + L] + s.grad[(5 + 2 * j) * nc + Rr]) * 0.5;
-          double sc = (UR[2 + j] - UL[2 + j] - dot2(a0, a1, l2r0, l2r1)) / l2sq;
-          gE[j][0] = a0 + l2r0 * sc;
-          gE[j][1] = a1 + l2r1 * sc;
-        }
-        double a0 = (s.gradT[L] + s.gradT[Rr]) * 0.5;
-        double a1 = (s.gradT[nc + L] + s.gradT[nc + Rr]) * 0.5;
-        double rightT = UR[1] / UR[0];
-        double leftT = UL[1] / UL[0];
-        double sc = (rightT - leftT - dot2(a0, a1, l2r0, l2r1)) / l2sq;
-        gE[2][0] = a0 + l2r0 * sc;
-        gE[2][1] = a1 + l2r1 * sc;
-      } else {
-        gE[0][0] = s.grad[4 * nc + c];
-        gE[0][1] = s.grad[5 * nc + c];
-        gE[1][0] = s.grad[6 * nc + c];
-        gE[1][1] = s.grad[7 * nc + c];
-        if (o.nb == kWallD) {
-          gE[2][0] = 0;
-          gE[2][1] = 0;
-        } else {
-          gE[2][0] = s.gradT[c];
-          gE[2][1] = s.gradT[nc + c];
-        }
-      }
-      double taoxx = efficient * (2.0 * gE[0][0] - gE[1][1]);
-      double taoyy = efficient * (2.0 * gE[1][1] - gE[0][0]);
-      double taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
-      double qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
-      double qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
-      double Fv[4];
-      Fv[0] = 0.0;
-      Fv[1] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
-      Fv[2] = (taoxx * n0 + taoxy * n1) * A;
-      Fv[3] = (taoxy * n0 + taoyy * n1) * A;
-      if (o.left) {
-        for (int j = 0; j < 4; j++) Rc[j] -= Fv[j];
-      } else {
-        for (int j = 0; j < 4; j++) Rc[j] += Fv[j];
-      }
-      if (F::sst) {
-        // face turbulence values (SSTKWStrategy.h:616-659)
-        double Vn = dot2(Ue[2], Ue[3], n0, n1);
-        double te[2], tnb[2] = {0, 0};
-        if (o.nb >= 0) {
-          tnb[0] = s.tU[o.nb];
-          tnb[1] = s.tU[nc + o.nb];
-          te[0] = o.left ? 0.5 * (tUc[0] + tnb[0]) : 0.5 * (tnb[0] + tUc[0]);
-          te[1] = o.left ? 0.5 * (tUc[1] + tnb[1]) : 0.5 * (tnb[1] + tUc[1]);
-        } else if (o.nb == kWallD) {
-          double dis = m.wdist[c];
-          te[0] = 0.0;
-          te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
-        } else if (Vn > 0.0) {
-          te[0] = tUc[0];
-          te[1] = tUc[1];
-        } else {
-          te[0] = k.fsnd_turb[0];
-          te[1] = k.fsnd_turb[1];
-        }
-        // convective (SolveConvectiveTFlux :783-831)
-        double Fc[2];
-        if (o.nb >= 0) {
-          double Vp = 0.5 * (Vn + fabs(Vn));
-          double Vs = 0.5 * (Vn - fabs(Vn));
-          const double *tl = o.left ? tUc : tnb;
-          const double *tr = o.left ? tnb : tUc;
-          double rl = o.left ? Uc[0] : Unb[0], rr = o.left ? Unb[0] : Uc[0];
-          for (int q = 0; q < 2; q++) Fc[q] = tl[q] * rl * Vp * A + tr[q] * rr * Vs * A;
-        } else {
-          for (int q = 0; q < 2; q++) Fc[q] = Vn * Ue[0] * te[q] * A;
-        }
-        // diffusive (SolveViscousTFlux :834-920)
-        double face_f1;
-        double gq[2][2];
-        if (o.nb >= 0) {
-          double l0 = m.ccx[Rr] - m.ccx[L], l1 = m.ccy[Rr] - m.ccy[L];
-          const double *lt = o.left ? tUc : tnb;
-          const double *rt = o.left ? tnb : tUc;
-          double f1l = o.left ? F1c : s.F1[o.nb], f1r = o.left ? s.F1[o.nb] : F1c;
-          face_f1 = 0.5 * (f1l + f1r);
-          for (int q = 0; q < 2; q++) {
-            gq[q][0] = (s.tgrad[(2 * q) * nc + L] + s.tgrad[(2 * q) * nc + Rr]) * 0.5;
-            gq[q][1] = (s.tgrad[(2 * q + 1) * nc + L] + s.tgrad[(2 * q + 1) * nc + Rr]) * 0.5;
-          }
-          double l2 = dot2(l0, l1, l0, l1);
-          for (int q = 0; q < 2; q++) {
-            gq[q][0] = gq[q][0] + ((rt[q] - lt[q]) - (gq[q][0] * l0 + gq[q][1] * l1)) * l0 / l2;
-            gq[q][1] = gq[q][1] + ((rt[q] - lt[q]) - (gq[q][0] * l0 + gq[q][1] * l1)) * l1 / l2;
-          }
-        } else {
-          face_f1 = F1c;
-          for (int q = 0; q < 2; q++) {
-            gq[q][0] = tgc[2 * q];
-            gq[q][1] = tgc[2 * q + 1];
-          }
-        }
-        double sigma_k = face_f1 * SST_SK1 + (1 - face_f1) * SST_SK2;
-        double sigma_w = face_f1 * SST_SW1 + (1 - face_f1) * SST_SW2;
-        double txx_k = (mue + sigma_k * mutf) * gq[0][0];
-        double tyy_k = (mue + sigma_k * mutf) * gq[0][1];
-        double txx_w = (mue + sigma_w * mutf) * gq[1][0];
-        double tyy_w = (mue + sigma_w * mutf) * gq[1][1];
-        double Fv0 = txx_k * n0 + tyy_k * n1;
-        double Fv1 = txx_w * n0 + tyy_w * n1;
-        Fv0 *= A;
-        Fv1 *= A;
-        if (o.left) {
-          trc[0] += Fc[0] - Fv0;
-          trc[1] += Fc[1] - Fv1;
-        } else {
-          trc[0] -= Fc[0] - Fv0;
-          trc[1] -= Fc[1] - Fv1;
-        }
-      }
-    }
-    if (F::sst) {
-      s.tR[c] = trc[0];
-      s.tR[nc + c] = trc[1];
+    double a0 = (s.gradT[L] + s.gradT[Rr]) * 0.5;
+    double a1 = (s.gradT[nc + L] + s.gradT[nc + Rr]) * 0.5;
+    double rightT = UR[1] / UR[0];
+    double leftT = UL[1] / UL[0];
+    double sc = (rightT - leftT - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+    gE[2][0] = a0 + l2r0 * sc;
+    gE[2][1] = a1 + l2r1 * sc;
+  } else {
+    gE[0][0] = s.grad[4 * nc + L];
+    gE[0][1] = s.grad[5 * nc + L];
+    gE[1][0] = s.grad[6 * nc + L];
+    gE[1][1] = s.grad[7 * nc + L];
+    if (Rr == kWallD) {
+      gE[2][0] = 0;
+      gE[2][1] = 0;
+    } else {
+      gE[2][0] = s.gradT[L];
+      gE[2][1] = s.gradT[nc + L];
     }
   }
-  store4(s.R, nc, c, Rc);
+  double taoxx = efficient * (2.0 * gE[0][0] - gE[1][1]);
+  double taoyy = efficient * (2.0 * gE[1][1] - gE[0][0]);
+  double taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
+  double qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
+  double qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
+  s.Fv[f] = 0.0;
+  s.Fv[nf + f] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
+  s.Fv[2 * nf + f] = (taoxx * n0 + taoxy * n1) * A;
+  s.Fv[3 * nf + f] = (taoxy * n0 + taoyy * n1) * A;
+  if (!F::sst) return;
+  // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
+  const double Vn = dot2(Ue[2], Ue[3], n0, n1);
+  double tl[2] = {s.tU[L], s.tU[nc + L]}, tr[2] = {0, 0}, te[2];
+  if (inner) {
+    tr[0] = s.tU[Rr];
+    tr[1] = s.tU[nc + Rr];
+    te[0] = 0.5 * (tl[0] + tr[0]);
+    te[1] = 0.5 * (tl[1] + tr[1]);
+  } else if (Rr == kWallD) {
+    double dis = m.wdist[L];
+    te[0] = 0.0;
+    te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
+  } else if (Vn > 0.0) {
+    te[0] = tl[0];
+    te[1] = tl[1];
+  } else {
+    te[0] = k.fsnd_turb[0];
+    te[1] = k.fsnd_turb[1];
+  }
+  // convective (SolveConvectiveTFlux :783-831)
+  double Fct[2];
+  if (inner) {
+    double Vp = 0.5 * (Vn + fabs(Vn));
+    double Vs = 0.5 * (Vn - fabs(Vn));
+    for (int q = 0; q < 2; q++) Fct[q] = tl[q] * UL[0] * Vp * A + tr[q] * UR[0] * Vs * A;
+  } else {
+    for (int q = 0; q < 2; q++) Fct[q] = Vn * Ue[0] * te[q] * A;
+  }
+  // diffusive (SolveViscousTFlux :834-920)
+  double face_f1;
+  double gq[2][2];
+  if (inner) {
+    face_f1 = 0.5 * (s.F1[L] + s.F1[Rr]);
+    for (int q = 0; q < 2; q++) {
+      gq[q][0] = (s.tgrad[(2 * q) * nc + L] + s.tgrad[(2 * q) * nc + Rr]) * 0.5;
+      gq[q][1] = (s.tgrad[(2 * q + 1) * nc + L] + s.tgrad[(2 * q + 1) * nc + Rr]) * 0.5;
+    }
+    for (int q = 0; q < 2; q++) {
+      gq[q][0] = gq[q][0] + ((tr[q] - tl[q]) - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r0 / l2sq;
+      gq[q][1] = gq[q][1] + ((tr[q] - tl[q]) - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r1 / l2sq;
+    }
+  } else {
+    face_f1 = s.F1[L];
+    for (int q = 0; q < 2; q++) {
+      gq[q][0] = s.tgrad[(2 * q) * nc + L];
+      gq[q][1] = s.tgrad[(2 * q + 1) * nc + L];
+    }
+  }
+  double sigma_k = face_f1 * SST_SK1 + (1 - face_f1) * SST_SK2;
+  double sigma_w = face_f1 * SST_SW1 + (1 - face_f1) * SST_SW2;
+  double txx_k = (mue + sigma_k * mutf) * gq[0][0];
+  double tyy_k = (mue + sigma_k * mutf) * gq[0][1];
+  double txx_w = (mue + sigma_w * mutf) * gq[1][0];
+  double tyy_w = (mue + sigma_w * mutf) * gq[1][1];
+  double Fv0 = txx_k * n0 + tyy_k * n1;
+  double Fv1 = txx_w * n0 + tyy_w * n1;
+  Fv0 *= A;
+  Fv1 *= A;
+  s.Ft[f] = Fct[0] - Fv0;
+  s.Ft[nf + f] = Fct[1] - Fv1;
 }
 
 // ---------------------------------------------------------------------------
-// k_sweep: one wave of the LU-SGS forward (FWD) or backward sweep.
-// LUSGSIntegrator::SolveForwardSweep/SolveBackwardSweep (LUSGS_Integrator.cpp:492-561),
-// SolveDF/SolveViscousFlux (TopologyHolderStrategy.cpp:143-323), SolveTime
-// (Euler2D.h:67-77, ViscousFlow2D.h:223-242), SolveDiag (:190-215); SST
-// SolveDiag/SolveDF (SSTKWStrategy.h:170-192,251-417, deviations D2/D4).
-// The backward wave also applies UpdatePrimitiveVariable
-// (LUSGS_Integrator.cpp:242-255) into the second state buffer.
+// k_cell: residual of one cell as the reference's scatter order gives it --
+// convective face fluxes in ascending face order, then the viscous pass
+// (R_L -= Fv, R_R += Fv, LaminarFluxStrategy.h:192-210), SST source then
+// faces (SolveTotalTFlux, SSTKWStrategy.h:969-1013, source sign -Q*V) -- and
+// the per-cell LU-SGS factors: dt (SolveTime, Euler2D.h:67-77 /
+// ViscousFlow2D.h:223-242), D (SolveDiag, TopologyHolderStrategy.cpp:190-215),
+// the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
 // ---------------------------------------------------------------------------
-template <int FLOW, bool FWD>
-__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
-                                                 const int *__restrict__ cells, int count) {
+template <int FLOW>
+__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k) {
   using F = Flow<FLOW>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const int c = cells[i];
-  const int nc = m.nc;
-  const double Gamma = USG_GAMMA, OMEGAN = 1.5;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc, nf = m.nf;
+  if (c >= nc) return;
+  int fs[4];
+  int ns = 0;
+  for (int sl = 0; sl < 4; sl++) {
+    int e = m.cf[sl * nc + c];
+    if (e == kNoSlot) break;
+    fs[ns++] = e;
+  }
+  double Rc[4] = {0, 0, 0, 0};
+  for (int i = 0; i < ns; i++) {
+    const int e = fs[i];
+    const int f = e >= 0 ? e : -e - 1;
+    if (e >= 0) {
+      for (int j = 0; j < 4; j++) Rc[j] += s.Fc[j * nf + f];
+    } else {
+      for (int j = 0; j < 4; j++) Rc[j] -= s.Fc[j * nf + f];
+    }
+  }
+  if (F::visc) {
+    for (int i = 0; i < ns; i++) {
+      const int e = fs[i];
+      const int f = e >= 0 ? e : -e - 1;
+      if (e >= 0) {
+        for (int j = 0; j < 4; j++) Rc[j] -= s.Fv[j * nf + f];
+      } else {
+        for (int j = 0; j < 4; j++) Rc[j] += s.Fv[j * nf + f];
+      }
+    }
+  }
+  store4(s.R, nc, c, Rc);
+  const double OMEGAN = 1.5;
   const double V = m.vol[c];
   const double lamc = s.lamc[c];
   const double lamv = F::visc ? s.lamv[c] : 0.0;
@@ -506,13 +467,75 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
     dtc = k.cfl * V / (lamc + 4.0 * lamv);
   else
     dtc = k.cfl * V / (lamc);
+  s.dt[c] = dtc;
   double diag = V / dtc + 0.5 * OMEGAN * lamc;
   if (F::visc) diag += lamv;
-  double mprc = 0;
+  s.D[c] = diag;
   if (F::visc) {
-    mprc = s.mu[c] / k.pr;
+    double mprc = s.mu[c] / k.pr;
     if (F::sst) mprc = mprc + s.mut[c] / k.prt;
+    s.mprc[c] = mprc;
   }
+  if (!F::sst) return;
+  double tUc[2] = {s.tU[c], s.tU[nc + c]};
+  double tgc[4];
+  for (int q = 0; q < 4; q++) tgc[q] = s.tgrad[q * nc + c];
+  const double F1c = s.F1[c], mutc = s.mut[c];
+  const double rho = s.U[c];
+  double gux = s.grad[4 * nc + c], guy = s.grad[5 * nc + c];
+  double gvx = s.grad[6 * nc + c], gvy = s.grad[7 * nc + c];
+  double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
+  double ddls1 = (guy + gvx) * (guy + gvx);
+  double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
+  double P = (ddls0 + ddls1) * mutc - ddls2;
+  double rdiv = dot2(tgc[0], tgc[1], tgc[2], tgc[3]);
+  double Cw = F1c * SST_CW1 + (1.0 - F1c) * SST_CW2;
+  double beta = F1c * SST_BETA1 + (1.0 - F1c) * SST_BETA2;
+  double kk = tUc[0], w = tUc[1];
+  double Q0 = P - SST_BSTAR * rho * kk * w;
+  double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
+  double trc[2] = {-(Q0 * V), -(Q1 * V)};
+  for (int i = 0; i < ns; i++) {
+    const int e = fs[i];
+    const int f = e >= 0 ? e : -e - 1;
+    if (e >= 0) {
+      trc[0] += s.Ft[f];
+      trc[1] += s.Ft[nf + f];
+    } else {
+      trc[0] -= s.Ft[f];
+      trc[1] -= s.Ft[nf + f];
+    }
+  }
+  s.tR[c] = trc[0];
+  s.tR[nc + c] = trc[1];
+  // SSTkomegaModel::SolveDiag
+  double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
+  s.tD[c] = diagt + 2.0 * SST_BSTAR * w * V;
+  double gradVal = rdiv;
+  s.tD[nc + c] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - F1c) * gradVal / (w * w));
+}
+
+// ---------------------------------------------------------------------------
+// k_sweep: one wave of the LU-SGS forward (FWD) or backward sweep.
+// LUSGSIntegrator::SolveForwardSweep/SolveBackwardSweep (LUSGS_Integrator.cpp:492-561),
+// SolveDF/SolveViscousFlux (TopologyHolderStrategy.cpp:143-323); SST
+// SolveDF (SSTKWStrategy.h:251-417, deviations D2/D4).  A wave is either a
+// contiguous cell range [start, start+count) (colour-blocked numbering) or a
+// slice of an index list.  The backward wave also applies
+// UpdatePrimitiveVariable (LUSGS_Integrator.cpp:242-255) into the second
+// state buffer, and the SST update (SSTKWStrategy.h:194-223).
+// ---------------------------------------------------------------------------
+template <int FLOW, bool FWD>
+__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
+                                                 const int *__restrict__ cells, int start, int count) {
+  using F = Flow<FLOW>;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int c = cells ? cells[start + i] : start + i;
+  const int nc = m.nc;
+  const double Gamma = USG_GAMMA, OMEGAN = 1.5;
+  const double diag = s.D[c];
+  const double mprc = F::visc ? s.mprc[c] : 0.0;
   const double *dsrc = FWD ? s.dW1 : s.dW;
   const double *tsrc = FWD ? s.tdW1 : s.tdW;
   double dFi[4] = {0, 0, 0, 0};
@@ -566,21 +589,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
       }
     }
   }
-  double tD[2] = {0, 0};
-  if (F::sst) {
-    // SSTkomegaModel::SolveDiag
-    double w = s.tU[nc + c];
-    double f1 = s.F1[c];
-    double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
-    tD[0] = diagt + 2.0 * SST_BSTAR * w * V;
-    double gradVal = dot2(s.tgrad[c], s.tgrad[nc + c], s.tgrad[2 * nc + c], s.tgrad[3 * nc + c]);
-    double beta = f1 * SST_BETA1 + (1 - f1) * SST_BETA2;
-    tD[1] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - f1) * gradVal / (w * w));
-  }
   if (FWD) {
     for (int j = 0; j < 4; j++) s.dW1[j * nc + c] = (-s.R[j * nc + c] - dFi[j]) / diag;
     if (F::sst)
-      for (int q = 0; q < 2; q++) s.tdW1[q * nc + c] = (-s.tR[q * nc + c] - tFi[q]) / tD[q];
+      for (int q = 0; q < 2; q++) s.tdW1[q * nc + c] = (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
   } else {
     double dW[4], Wn[4], Un[4];
     for (int j = 0; j < 4; j++) {
@@ -594,12 +606,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
     if (F::sst) {
       double tdw[2], tw[2];
       for (int q = 0; q < 2; q++) {
-        tdw[q] = s.tdW1[q * nc + c] - tFi[q] / tD[q];
+        tdw[q] = s.tdW1[q * nc + c] - tFi[q] / s.tD[q * nc + c];
         s.tdW[q * nc + c] = tdw[q];
         tw[q] = s.tW[q * nc + c] + tdw[q];
-        s.tWn[q * nc + c] = tw[q];
       }
-      // SSTkomegaModel::UpdatePrimitiveVariables (SSTKWStrategy.h:194-223)
       double rho = Un[0];
       double tu[2] = {tw[0] / rho, tw[1] / rho};
       // floor at 1e-4 of the free stream (the clip kept commented out at
@@ -609,9 +619,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
         if (!(tu[q] >= fl)) {
           tu[q] = fl;
           tw[q] = rho * fl;
-          s.tWn[q * nc + c] = tw[q];
         }
       }
+      s.tWn[c] = tw[0];
+      s.tWn[nc + c] = tw[1];
       s.tUn[c] = tu[0];
       s.tUn[nc + c] = tu[1];
       double curl = fabs(s.grad[6 * nc + c] - s.grad[5 * nc + c]);
@@ -630,9 +641,9 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(const DevMesh m, const DevS
   const double V = m.vol[c];
   if (stage == 0) {
     double dtc = visc_sst ? cfl * V / (s.lamc[c] + 4.0 * s.lamv[c]) : cfl * V / (s.lamc[c]);
-    s.dt[c] = dtc;
+    s.dt_rk[c] = dtc;
   }
-  const double dtc = s.dt[c];
+  const double dtc = s.dt_rk[c];
   double Wn[4], Un[4];
   for (int j = 0; j < 4; j++) Wn[j] = s.W0[j * nc + c] + alpha * dtc * (-s.R[j * nc + c]) / V;
   prim_of(Wn, Un);
@@ -861,11 +872,12 @@ static void launch_residual(usg_ctx *c) {
     }
     hipEventRecord(c->ev[c->ev_used].first, c->stream);
   }
-  hipLaunchKernelGGL(k_flux<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  hipLaunchKernelGGL(k_face<FLOW>, dim3(grid(c->hm.nf)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
   if (c->timing) {
     hipEventRecord(c->ev[c->ev_used].second, c->stream);
     c->ev_used++;
   }
+  hipLaunchKernelGGL(k_cell<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
 }
 
 static void launch_residual_any(usg_ctx *c) {
@@ -878,17 +890,33 @@ static void launch_residual_any(usg_ctx *c) {
 
 template <int FLOW>
 static void launch_sweeps(usg_ctx *c) {
-  const auto &fo = c->hm.fwd_off;
+  const HostMesh &h = c->hm;
+  if (h.use_ranges) {
+    const auto &r = h.ranges;
+    const int nw = (int)r.size() - 1;
+    for (int w = 0; w < nw; w++) {
+      int cnt = r[w + 1] - r[w];
+      hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                         (const int *)nullptr, r[w], cnt);
+    }
+    for (int w = nw - 1; w >= 0; w--) {
+      int cnt = r[w + 1] - r[w];
+      hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                         (const int *)nullptr, r[w], cnt);
+    }
+    return;
+  }
+  const auto &fo = h.fwd_off;
   for (size_t w = 0; w + 1 < fo.size(); w++) {
     int cnt = fo[w + 1] - fo[w];
     hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_fwd + fo[w], cnt);
+                       (const int *)c->d_fwd, fo[w], cnt);
   }
-  const auto &bo = c->hm.bwd_off;
+  const auto &bo = h.bwd_off;
   for (size_t w = 0; w + 1 < bo.size(); w++) {
     int cnt = bo[w + 1] - bo[w];
     hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_bwd + bo[w], cnt);
+                       (const int *)c->d_bwd, bo[w], cnt);
   }
 }
 
@@ -1009,6 +1037,8 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(upload(c, (double **)&dm.wdist, h.wdist.data(), nc));
   CK(upload(c, (int **)&dm.cf, cf.data(), cf.size()));
   CK(upload(c, (int **)&dm.cnb, cnb.data(), cnb.size()));
+  CK(upload(c, (int **)&dm.flc, h.flc.data(), nf));
+  CK(upload(c, (int **)&dm.frc, h.frc.data(), nf));
   CK(upload(c, (double **)&dm.fnx, fnx.data(), nf));
   CK(upload(c, (double **)&dm.fny, fny.data(), nf));
   CK(upload(c, (double **)&dm.fA, h.fa.data(), nf));
@@ -1033,12 +1063,17 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(dalloc(c, &s.dW1, 4 * (size_t)nc));
   CK(dalloc(c, &s.dW, 4 * (size_t)nc));
   CK(dalloc(c, &s.dt, nc));
+  CK(dalloc(c, &s.dt_rk, nc));
   CK(dalloc(c, &s.W0, 4 * (size_t)nc));
+  CK(dalloc(c, &s.Fc, 4 * (size_t)nf));
+  CK(dalloc(c, &s.D, nc));
   CK(dalloc(c, &s.err, 2));
   if (visc) {
     CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
     CK(dalloc(c, &s.lamv, nc));
     CK(dalloc(c, &s.mu, nc));
+    CK(dalloc(c, &s.Fv, 4 * (size_t)nf));
+    CK(dalloc(c, &s.mprc, nc));
   }
   if (sst) {
     CK(dalloc(c, &s.tU, 2 * (size_t)nc));
@@ -1053,6 +1088,8 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
     CK(dalloc(c, &s.tR, 2 * (size_t)nc));
     CK(dalloc(c, &s.tdW1, 2 * (size_t)nc));
     CK(dalloc(c, &s.tdW, 2 * (size_t)nc));
+    CK(dalloc(c, &s.Ft, 2 * (size_t)nf));
+    CK(dalloc(c, &s.tD, 2 * (size_t)nc));
   }
   CK(dalloc(c, &c->d_partial, c->nblk));
   CK(dalloc(c, &c->d_force, 2));
@@ -1230,8 +1267,9 @@ int usg_forces(usg_ctx *c, double *cl, double *cd) {
 
 int usg_sweep_waves(usg_ctx *c, int *nfwd, int *nbwd) {
   if (!c || !nfwd || !nbwd) return fail(USG_ERR_ARG, "null argument");
-  *nfwd = (int)c->hm.fwd_off.size() - 1;
-  *nbwd = (int)c->hm.bwd_off.size() - 1;
+  const HostMesh &h = c->hm;
+  *nfwd = h.use_ranges ? (int)h.ranges.size() - 1 : (int)h.fwd_off.size() - 1;
+  *nbwd = h.use_ranges ? (int)h.ranges.size() - 1 : (int)h.bwd_off.size() - 1;
   return USG_OK;
 }
 

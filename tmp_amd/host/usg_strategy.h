@@ -1,0 +1,302 @@
+// C++ host adapters over the C ABI (include/usg.h), mirroring the reference's
+// solver-facing interfaces so a driver written against them switches to the
+// gfx950 path by changing the constructed types:
+//   UnstructReader           reader/UnstructReader.h:88-285 (serial Cobalt reader)
+//   TopologyHolderStrategy   UnstructSolver/TopologyHolderStrategy.h:15-336
+//   TimeIntegrator           UnstructSolver/TimeIntegrator.h:5-27
+//   LUSGSIntegrator          UnstructSolver/LUSGS_Integrator.h:8-82
+//   RungeKuttaStrategy       UnstructSolver/RungeKuttaStrategy.h:5-221
+//   WrapSolver               UnstructSolver/SolverWrapper.h:7-95
+// Getters return host mirrors in the reference's AoS layout (double** per
+// mesh), refreshed from the device on each call.  Errors throw
+// std::runtime_error with the library's message, as the reference throws.
+// Header-only C++17; link against libusg.so.
+#pragma once
+
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/usg.h"
+
+namespace usg {
+
+inline void check(int rc) {
+  if (rc != USG_OK) throw std::runtime_error(std::string("usg: ") + usg_last_error());
+}
+
+// ---------------------------------------------------------------------------
+// Mesh input: the serial Cobalt reader's conventions (UnstructReader.h:124-262)
+// ---------------------------------------------------------------------------
+struct MeshData {
+  int nn = 0, nf = 0, nc = 0;
+  std::vector<double> xy;
+  std::vector<int> fnode, flc, frc;
+  usg_mesh_desc desc() const {
+    return usg_mesh_desc{nn, nf, nc, xy.data(), fnode.data(), flc.data(), frc.data()};
+  }
+};
+
+class UnstructReader {
+ public:
+  void setDim(int ndim) {
+    if (ndim != 2) throw std::runtime_error("Not implemented yet\n");
+  }
+  void setOverSign(int sign) { signOver_ = sign; }
+  void setWallSign(int sign) { signWall_ = sign; }
+  void addFile(const std::string &f) { files_.push_back(f); }
+  int nMesh() const { return (int)meshes_.size(); }
+  const MeshData &mesh(int i) const { return meshes_.at(i); }
+
+  void readAll() {
+    meshes_.clear();
+    for (const auto &f : files_) meshes_.push_back(read(f));
+  }
+
+ private:
+  MeshData read(const std::string &filename) {
+    std::ifstream fin(filename);
+    if (!fin.is_open()) throw std::runtime_error("Open file Failure\n");
+    MeshData m;
+    int skip;
+    fin >> skip >> skip >> skip;
+    fin >> m.nn >> m.nf >> m.nc >> skip >> skip;
+    m.xy.resize(2 * (size_t)m.nn);
+    for (auto &v : m.xy) fin >> v;
+    m.fnode.resize(2 * (size_t)m.nf);
+    m.flc.resize(m.nf);
+    m.frc.resize(m.nf);
+    for (int i = 0; i < m.nf; i++) {
+      int cnt, lC, rC;
+      fin >> cnt;
+      if (cnt != 2) throw std::runtime_error("2-D faces must have two nodes\n");
+      fin >> m.fnode[2 * i] >> m.fnode[2 * i + 1] >> lC >> rC;
+      m.fnode[2 * i] -= 1;
+      m.fnode[2 * i + 1] -= 1;
+      if (lC <= 0) throw std::runtime_error("Left cell can't be negative\n");
+      m.flc[i] = lC - 1;
+      if (rC > 0)
+        m.frc[i] = rC - 1;
+      else if (rC == signWall_)
+        m.frc[i] = USG_WALL;
+      else if (rC == signOver_)
+        m.frc[i] = USG_FARFIELD;
+      else
+        throw std::runtime_error(" Not a valid boundary type\n");
+    }
+    if (!fin) throw std::runtime_error("truncated .grd file\n");
+    return m;
+  }
+  int signWall_ = -2, signOver_ = -3;
+  std::vector<std::string> files_;
+  std::vector<MeshData> meshes_;
+};
+
+// ---------------------------------------------------------------------------
+// Flow strategy: one GPU context per mesh (the reference's d_nmesh blocks)
+// ---------------------------------------------------------------------------
+class GpuFlowStrategy {
+ public:
+  // flow: USG_EULER ("Euler2D") / USG_LAMINAR ("ViscousFlow2D") / USG_SST
+  explicit GpuFlowStrategy(int flow, int precision = USG_FP64) : flow_(flow), precision_(precision) {}
+  ~GpuFlowStrategy() {
+    for (auto *c : ctx_) usg_ctx_destroy(c);
+  }
+  GpuFlowStrategy(const GpuFlowStrategy &) = delete;
+  GpuFlowStrategy &operator=(const GpuFlowStrategy &) = delete;
+
+  // TopologyHolderStrategy::set_fs_variable2 (TopologyHolderStrategy.cpp:325-351);
+  // density and pressure are fixed at 1.225 / 101325 as WrapSolver passes them.
+  void set_fs_variable2(double Ma, double AOA, double /*density*/, double /*pressure*/, double Re) {
+    ma_ = Ma;
+    aoa_ = AOA;
+    re_ = Re;
+  }
+  // TopologyHolderStrategy::initializeStrategy
+  void initializeStrategy(const UnstructReader &rd) {
+    for (int i = 0; i < rd.nMesh(); i++) {
+      usg_mesh_desc d = rd.mesh(i).desc();
+      usg_params p{flow_, precision_, ma_, aoa_, re_, cfl_lusgs_, cfl_rk_};
+      usg_ctx *c = nullptr;
+      check(usg_ctx_create(&d, &p, &c));
+      ctx_.push_back(c);
+      nc_.push_back(d.nc);
+    }
+    mirrors_.assign(16, std::vector<std::vector<double>>(ctx_.size()));
+    ptrs_.assign(16, std::vector<double *>(ctx_.size(), nullptr));
+  }
+  void setCFL(double lusgs, double rk) {
+    cfl_lusgs_ = lusgs;
+    cfl_rk_ = rk;
+  }
+  int getNMesh() const { return (int)ctx_.size(); }
+  int getNEquation() const { return 4; }
+  int nCells(int m) const { return nc_.at(m); }
+  bool isInvicid() const { return flow_ == USG_EULER; }
+  bool isLaminar() const { return flow_ != USG_SST; }
+
+  // Euler2D::initializeData (Euler2D.cpp:153-171)
+  void initializeData() {
+    for (auto *c : ctx_) check(usg_init_freestream(c));
+  }
+  // warm start from AoS primitives of mesh m (and (k, omega) for SST)
+  void setPrimitive(int m, const double *U, const double *turb = nullptr) {
+    check(usg_set_prim(ctx_.at(m), U, turb));
+  }
+  // preprocessAdvanceSerial (Euler2D.h:133-140 / ViscousFlow2D.h:192-212)
+  void preprocessAdvanceSerial(int /*istage*/) {
+    for (auto *c : ctx_) check(usg_residual(c));
+  }
+
+  // getters of TopologyHolderStrategy.h:242-303 (host mirrors, AoS)
+  double **getU() { return mirror(USG_F_PRIM, 4); }
+  double **getResidual() { return mirror(USG_F_RES, 4); }
+  void **getGradientPrimitive() { return (void **)mirror(USG_F_GRAD, 8); }
+  double **getSpectrumConvective() { return mirror(USG_F_LAMC, 1); }
+  double **getSpectrumViscous() { return isInvicid() ? nullptr : mirror(USG_F_LAMV, 1); }
+  void **getGradientT() { return isInvicid() ? nullptr : (void **)mirror(USG_F_GRADT, 2); }
+  double **getStableTime() { return mirror(USG_F_DT, 1); }
+  double **getLimiter() { return mirror(USG_F_PHI, 4); }
+  double **getTurbulence() { return flow_ == USG_SST ? mirror(USG_F_TURB, 2) : nullptr; }
+
+  // integrated wall force of mesh m
+  void forces(int m, double &cl, double &cd) { check(usg_forces(ctx_.at(m), &cl, &cd)); }
+  double residualNorm(int m) {
+    double r;
+    check(usg_res_l2(ctx_.at(m), &r));
+    return r;
+  }
+  usg_ctx *context(int m) { return ctx_.at(m); }
+
+ private:
+  double **mirror(int field, int width) {
+    for (size_t m = 0; m < ctx_.size(); m++) {
+      auto &buf = mirrors_[field][m];
+      buf.resize((size_t)width * nc_[m]);
+      check(usg_get_field(ctx_[m], field, buf.data(), (long)buf.size()));
+      ptrs_[field][m] = buf.data();
+    }
+    return ptrs_[field].data();
+  }
+  int flow_, precision_;
+  double ma_ = 0.5, aoa_ = 0.0, re_ = 1e6, cfl_lusgs_ = 400, cfl_rk_ = 0.8;
+  std::vector<usg_ctx *> ctx_;
+  std::vector<int> nc_;
+  std::vector<std::vector<std::vector<double>>> mirrors_;
+  std::vector<std::vector<double *>> ptrs_;
+};
+
+// ---------------------------------------------------------------------------
+// Time integrators (TimeIntegrator.h:5-27)
+// ---------------------------------------------------------------------------
+class GpuTimeIntegrator {
+ public:
+  virtual ~GpuTimeIntegrator() = default;
+  void initializeStrategy(GpuFlowStrategy *s) { strategy_ = s; }
+  virtual void initializeData() { strategy_->initializeData(); }
+  virtual void singleStep(int curStep) = 0;
+  // The reference's converged() falls off the end (LUSGS_Integrator.cpp:156-161);
+  // here: ||R_rho/V||_2 of the last step below tolerance * first step's.
+  virtual bool converged() {
+    if (first_ <= 0) return false;
+    return last_ <= tol_ * first_;
+  }
+  void setTolerance(double t) { tol_ = t; }
+  double lastResidual() const { return last_; }
+
+ protected:
+  void record(double r) {
+    if (first_ <= 0) first_ = r;
+    last_ = r;
+  }
+  GpuFlowStrategy *strategy_ = nullptr;
+  double first_ = -1, last_ = -1, tol_ = 1e-8;
+};
+
+// LUSGSIntegrator::singleStepSerial -> UpdateSerial (LUSGS_Integrator.cpp:124-129,228-255)
+class GpuLUSGSIntegrator : public GpuTimeIntegrator {
+ public:
+  void singleStep(int /*curStep*/) override { run(1); }
+  // n iterations in one call (one launch sequence, one synchronisation)
+  void run(int n) {
+    std::vector<double> h(n > 0 ? n : 1);
+    for (int m = 0; m < strategy_->getNMesh(); m++) check(usg_lusgs_step(strategy_->context(m), n, h.data()));
+    if (n > 0) record(h[n - 1]);
+  }
+};
+
+// RungeKuttaStrategy(5)::singleStepSerial (RungeKuttaStrategy.h:28-72)
+class GpuRK5Integrator : public GpuTimeIntegrator {
+ public:
+  void singleStep(int /*curStep*/) override {
+    double h = 0;
+    for (int m = 0; m < strategy_->getNMesh(); m++) check(usg_rk5_step(strategy_->context(m), 1, &h));
+    record(h);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// WrapSolver (SolverWrapper.h:7-95): string dispatch of flow and integrator
+// ---------------------------------------------------------------------------
+class WrapSolver {
+ public:
+  void setTopology(UnstructReader *rd) { reader_ = rd; }
+  void setParams(const std::string &flowT, const std::string &advanceT, const std::string & /*solveT*/,
+                 int MaxStep, double MinRes, double ma, double re, double *aoa) {
+    int flow;
+    if (flowT == "Euler2D")
+      flow = USG_EULER;
+    else if (flowT == "ViscousFlow2D")
+      flow = USG_LAMINAR;
+    else if (flowT == "SST2D")
+      flow = USG_SST;
+    else
+      throw std::runtime_error("Undefined Flow Type\n");
+    flow_.reset(new GpuFlowStrategy(flow));
+    if (advanceT == "RungeKutta5")
+      advance_.reset(new GpuRK5Integrator());
+    else if (advanceT == "LUSGS")
+      advance_.reset(new GpuLUSGSIntegrator());
+    else
+      throw std::runtime_error("Undefined Advance Type\n");
+    flow_->set_fs_variable2(ma, *aoa, 1.225, 101325, re);
+    maxStep_ = MaxStep;  // stored (the reference leaves d_maxStep uninitialised)
+    minRes_ = MinRes;
+    advance_->setTolerance(MinRes);
+  }
+  void init() {
+    flow_->initializeStrategy(*reader_);
+    advance_->initializeStrategy(flow_.get());
+    advance_->initializeData();
+  }
+  // Loop (SolverWrapper.h:71-90) without the Tecplot dumps; stops at
+  // MaximumTimeStep or when the relative residual reaches MinimumResidual.
+  int Loop(int report_every = 0) {
+    int i = 0;
+    for (; i < maxStep_; i++) {
+      advance_->singleStep(i);
+      if (report_every > 0 && i % report_every == 0)
+        std::cout << i << " " << advance_->lastResidual() << '\n';
+      if (advance_->converged()) {
+        i++;
+        break;
+      }
+    }
+    return i;
+  }
+  GpuFlowStrategy *flow() { return flow_.get(); }
+  GpuTimeIntegrator *advance() { return advance_.get(); }
+
+ private:
+  UnstructReader *reader_ = nullptr;
+  std::unique_ptr<GpuFlowStrategy> flow_;
+  std::unique_ptr<GpuTimeIntegrator> advance_;
+  int maxStep_ = 0;
+  double minRes_ = 0;
+};
+
+}  // namespace usg
