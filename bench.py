@@ -56,6 +56,27 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
     return m.nc * n / dt, n, dt
 
 
+def copy_calibration(mb):
+    """Runs the library's plain copy kernel (usg_copy_kernel) three times over
+    ``mb`` MB: FETCH_SIZE / WRITE_SIZE of this dispatch are ``mb`` MB each."""
+    import ctypes
+    import torch
+    from tmp_amd import usg
+    n = mb * 1024 * 1024 // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty(n, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    L = usg.lib()
+    for _ in range(3):
+        rc = L.usg_copy_kernel(ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(src.data_ptr()),
+                               ctypes.c_long(n * 8), None)
+        if rc:
+            raise RuntimeError(L.usg_last_error().decode())
+    torch.cuda.synchronize()
+    assert bool((dst == 1).all())
+    print(f"# copy kernel: 3 x {mb} MB", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,7 +85,12 @@ def main():
     ap.add_argument("--workload", default="sst1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run the plain copy kernel over MB megabytes (traffic-counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py: --gpus {a.gpus} is not supported; this build runs the path on one GPU "
               "(multi-GPU sharding is a design note only)", file=sys.stderr)

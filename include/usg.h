@@ -139,6 +139,10 @@ int usg_kernel_time(usg_ctx *ctx, double *total_ms, long *launches);
 int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const int *nw,
                       const int *donor, const double *w, void *hip_stream);
 
+/* Plain device copy kernel (n = bytes/8 doubles) on a stream: the reference
+ * kernel of known traffic used to calibrate HBM byte counters. */
+int usg_copy_kernel(void *dst, const void *src, long bytes, void *hip_stream);
+
 /* Last error text of the calling thread. */
 const char *usg_last_error(void);
 

@@ -758,6 +758,13 @@ __global__ __launch_bounds__(kBlock) void k_stencil(double *q, int nvar, int nre
   q[(long)recv[r] * nvar + kv] = acc;
 }
 
+// Plain copy of n doubles: the reference kernel for calibrating traffic counters.
+__global__ __launch_bounds__(kBlock) void k_copy(double *dst, const double *src, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace usg
 
 // ===========================================================================
@@ -1332,6 +1339,16 @@ int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const i
   hipStream_t stream = (hipStream_t)st;
   hipLaunchKernelGGL(k_stencil, dim3(grid(nrecv * nvar)), dim3(kBlock), 0, stream, (double *)dev_q, nvar, nrecv,
                      recv, nw, donor, w);
+  HIPCHK(hipGetLastError());
+  return USG_OK;
+}
+
+int usg_copy_kernel(void *dst, const void *src, long bytes, void *st) {
+  if (!dst || !src || bytes < 0 || bytes % 8) return fail(USG_ERR_ARG, "bad argument");
+  long n = bytes / 8;
+  int blocks = (int)std::min<long>((n + kBlock - 1) / kBlock, 16384);
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, (hipStream_t)st, (double *)dst, (const double *)src, n);
   HIPCHK(hipGetLastError());
   return USG_OK;
 }

@@ -16,12 +16,36 @@ def summary(db: str) -> list[tuple]:
     return rows
 
 
+def counters(db: str) -> dict:
+    """{kernel name: {counter: mean value per dispatch}} from a --pmc run."""
+    c = sqlite3.connect(db)
+    rows = list(c.execute("select kernel_name, counter_name, dispatch_id, value from counters_collection"))
+    c.close()
+    acc: dict = {}
+    for name, cn, d, v in rows:
+        acc.setdefault(name, {}).setdefault(cn, {})
+        acc[name][cn][d] = acc[name][cn].get(d, 0.0) + v
+    return {k: {cn: sum(dv.values()) / len(dv) for cn, dv in cs.items()} for k, cs in acc.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("db")
     ap.add_argument("--title", default="rocprofv3 kernel summary")
     ap.add_argument("--command", default="")
+    ap.add_argument("--pmc", action="store_true", help="the db holds --pmc counters")
     a = ap.parse_args()
+    if a.pmc:
+        print(f"# {a.title}\n")
+        if a.command:
+            print(f"Command: `{a.command}`\n")
+        cs = counters(a.db)
+        names = sorted({cn for v in cs.values() for cn in v})
+        print("| kernel | " + " | ".join(names) + " |")
+        print("|---|" + "---:|" * len(names))
+        for k, v in sorted(cs.items()):
+            print(f"| `{k[:60]}` | " + " | ".join(f"{v.get(n, float('nan')):.4g}" for n in names) + " |")
+        return
     print(f"# {a.title}\n")
     if a.command:
         print(f"Command: `{a.command}`\n")
