@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 
 namespace usg {
@@ -149,6 +150,96 @@ void waves(HostMesh &m) {
   m.use_ranges = m.ranges.size() - 1 == m.fwd_off.size() - 1;
 }
 
+uint64_t hilbert(uint32_t x, uint32_t y, int order) {
+  uint64_t d = 0;
+  for (uint32_t s = 1u << (order - 1); s > 0; s >>= 1) {
+    uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+    d += (uint64_t)s * s * ((3 * rx) ^ ry);
+    if (ry == 0) {
+      if (rx == 1) {
+        x = s - 1 - x;
+        y = s - 1 - y;
+      }
+      std::swap(x, y);
+    }
+  }
+  return d;
+}
+
+// Group `items` (ascending) by the part of each item, keeping the order.
+void by_part(const std::vector<int> &items, const std::vector<int> &item_part, std::vector<int> &out,
+             int *off) {
+  int cnt[kParts + 1] = {0};
+  for (int it : items) cnt[item_part[it] + 1]++;
+  for (int p = 0; p < kParts; p++) cnt[p + 1] += cnt[p];
+  for (int p = 0; p <= kParts; p++) off[p] = cnt[p];
+  std::vector<int> pos(cnt, cnt + kParts);
+  out.resize(items.size());
+  for (int it : items) out[pos[item_part[it]]++] = it;
+}
+
+void xcd_lists(HostMesh &m) {
+  const int nc = m.nc, nf = m.nf;
+  double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+  for (int i = 0; i < nc; i++) {
+    xmin = std::min(xmin, m.cc[2 * i]);
+    xmax = std::max(xmax, m.cc[2 * i]);
+    ymin = std::min(ymin, m.cc[2 * i + 1]);
+    ymax = std::max(ymax, m.cc[2 * i + 1]);
+  }
+  double span = std::max(std::max(xmax - xmin, ymax - ymin), 1e-300);
+  const int order = 16;
+  const double n = (double)((1u << order) - 1);
+  std::vector<std::pair<uint64_t, int>> key(nc);
+  for (int i = 0; i < nc; i++) {
+    uint32_t xi = (uint32_t)std::min(n, std::max(0.0, (m.cc[2 * i] - xmin) / span * n));
+    uint32_t yi = (uint32_t)std::min(n, std::max(0.0, (m.cc[2 * i + 1] - ymin) / span * n));
+    key[i] = {hilbert(xi, yi, order), i};
+  }
+  std::stable_sort(key.begin(), key.end());
+  m.part.assign(nc, 0);
+  for (int r = 0; r < nc; r++) m.part[key[r].second] = (int)((long long)r * kParts / nc);
+  std::vector<int> all(nc);
+  for (int i = 0; i < nc; i++) all[i] = i;
+  m.cell_off.assign(kParts + 1, 0);
+  by_part(all, m.part, m.cell_list, m.cell_off.data());
+  std::vector<int> fpart(nf), fall(nf);
+  for (int f = 0; f < nf; f++) {
+    fpart[f] = m.part[m.flc[f]];
+    fall[f] = f;
+  }
+  m.face_off.assign(kParts + 1, 0);
+  by_part(fall, fpart, m.face_list, m.face_off.data());
+  auto waves_by_part = [&](const std::vector<int> &cells, const std::vector<int> &woff, std::vector<int> &list,
+                           std::vector<int> &poff) {
+    const int nw = (int)woff.size() - 1;
+    list.assign(cells.size(), 0);
+    poff.assign((size_t)nw * (kParts + 1), 0);
+    for (int w = 0; w < nw; w++) {
+      std::vector<int> sub(cells.begin() + woff[w], cells.begin() + woff[w + 1]), out;
+      int off[kParts + 1];
+      by_part(sub, m.part, out, off);
+      std::copy(out.begin(), out.end(), list.begin() + woff[w]);
+      for (int p = 0; p <= kParts; p++) poff[(size_t)w * (kParts + 1) + p] = woff[w] + off[p];
+    }
+  };
+  if (m.use_ranges) {
+    std::vector<int> ident(nc);
+    for (int i = 0; i < nc; i++) ident[i] = i;
+    waves_by_part(ident, m.ranges, m.fwd_list, m.fwd_poff);
+    m.bwd_list = m.fwd_list;
+    // backward: the same ranges in reverse order
+    const int nw = (int)m.ranges.size() - 1;
+    m.bwd_poff.assign(m.fwd_poff.size(), 0);
+    for (int w = 0; w < nw; w++)
+      for (int p = 0; p <= kParts; p++)
+        m.bwd_poff[(size_t)w * (kParts + 1) + p] = m.fwd_poff[(size_t)(nw - 1 - w) * (kParts + 1) + p];
+  } else {
+    waves_by_part(m.fwd_cells, m.fwd_off, m.fwd_list, m.fwd_poff);
+    waves_by_part(m.bwd_cells, m.bwd_off, m.bwd_list, m.bwd_poff);
+  }
+}
+
 }  // namespace
 
 std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,
@@ -240,6 +331,7 @@ std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int 
     m.wdist[i] = d;
   }
   waves(m);
+  xcd_lists(m);
   return std::string();
 }
 

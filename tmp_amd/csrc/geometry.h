@@ -33,7 +33,19 @@ struct HostMesh {
   std::vector<int> ranges;
   bool use_ranges = false;
   std::vector<int> wall_faces;
+  // XCD-aware work lists: cells / faces / wave cells grouped by spatial part
+  // (kParts equal slices of the Hilbert curve of the cell centres; a face
+  // belongs to its left cell's part), ascending id inside a part.  Launches
+  // send part p's blocks to XCD p (blockIdx % kParts) so that neighbour
+  // gathers hit the XCD's own L2.
+  std::vector<int> part;                      // [nc]
+  std::vector<int> cell_list, cell_off;       // [nc], [kParts+1]
+  std::vector<int> face_list, face_off;       // [nf], [kParts+1]
+  std::vector<int> fwd_list, fwd_poff;        // [nc], [nwaves*(kParts+1)]
+  std::vector<int> bwd_list, bwd_poff;        // [nc], [nwaves*(kParts+1)]
 };
+
+constexpr int kParts = 8;  // XCDs of an MI355X
 
 // Validates and copies the mesh, then computes the derived geometry.
 // Returns an empty string on success, else the error text.

@@ -8,6 +8,21 @@
 namespace usg {
 
 constexpr int kNoSlot = -2147483647 - 1;  // empty face slot of a triangle
+constexpr int kXcd = 8;                    // XCDs (and spatial parts) of an MI355X
+
+// A work list split into kXcd parts: items[off[p] .. off[p+1]) belong to part
+// p.  Block b of a launch serves part b % kXcd (workgroups are dispatched to
+// the XCDs round-robin), chunk b / kXcd.
+struct XSlice {
+  int off[kXcd + 1];
+};
+__device__ __forceinline__ bool xcd_item(const int *__restrict__ items, const XSlice &sl, int &item) {
+  const int part = blockIdx.x % kXcd;
+  const int i = sl.off[part] + (int)(blockIdx.x / kXcd) * (int)blockDim.x + (int)threadIdx.x;
+  if (i >= sl.off[part + 1]) return false;
+  item = items[i];
+  return true;
+}
 constexpr int kWallD = -2;
 constexpr int kFarD = -3;
 

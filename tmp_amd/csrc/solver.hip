@@ -98,11 +98,12 @@ __device__ __forceinline__ double face_mut(const Consts &k, const DevState &s, i
 // LaminarFluxStrategy::computeMu (cells), SSTKWStrategy.h:616-722.
 // ---------------------------------------------------------------------------
 template <int FLOW>
-__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k) {
+__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k,
+                                                const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c;
+  if (!xcd_item(items, sl, c)) return;
   const int nc = m.nc;
-  if (c >= nc) return;
   const double Gamma = USG_GAMMA;
   double Uc[4];
   load4(s.U, nc, c, Uc);
@@ -268,11 +269,12 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
 // oriented out of the left cell.
 // ---------------------------------------------------------------------------
 template <int FLOW>
-__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k) {
+__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k,
+                                                const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  int f;
+  if (!xcd_item(items, sl, f)) return;
   const int nc = m.nc, nf = m.nf;
-  if (f >= nf) return;
   const double Gamma = USG_GAMMA;
   const int L = m.flc[f], Rr = m.frc[f];
   const double n0 = m.fnx[f], n1 = m.fny[f], A = m.fA[f];
@@ -424,11 +426,12 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
 // the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
 // ---------------------------------------------------------------------------
 template <int FLOW>
-__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k) {
+__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k,
+                                                const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int c;
+  if (!xcd_item(items, sl, c)) return;
   const int nc = m.nc, nf = m.nf;
-  if (c >= nc) return;
   int fs[4];
   int ns = 0;
   for (int sl = 0; sl < 4; sl++) {
@@ -519,19 +522,18 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
 // k_sweep: one wave of the LU-SGS forward (FWD) or backward sweep.
 // LUSGSIntegrator::SolveForwardSweep/SolveBackwardSweep (LUSGS_Integrator.cpp:492-561),
 // SolveDF/SolveViscousFlux (TopologyHolderStrategy.cpp:143-323); SST
-// SolveDF (SSTKWStrategy.h:251-417, deviations D2/D4).  A wave is either a
-// contiguous cell range [start, start+count) (colour-blocked numbering) or a
-// slice of an index list.  The backward wave also applies
+// SolveDF (SSTKWStrategy.h:251-417, deviations D2/D4).  A wave is a slice of
+// a part-grouped cell list (a colour block of a colour-blocked numbering, or
+// a dependency level).  The backward wave also applies
 // UpdatePrimitiveVariable (LUSGS_Integrator.cpp:242-255) into the second
 // state buffer, and the SST update (SSTKWStrategy.h:194-223).
 // ---------------------------------------------------------------------------
 template <int FLOW, bool FWD>
 __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
-                                                 const int *__restrict__ cells, int start, int count) {
+                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  const int c = cells ? cells[start + i] : start + i;
+  int c;
+  if (!xcd_item(items, sl, c)) return;
   const int nc = m.nc;
   const double Gamma = USG_GAMMA, OMEGAN = 1.5;
   const double diag = s.D[c];
@@ -771,6 +773,7 @@ __global__ __launch_bounds__(kBlock) void k_copy(double *dst, const double *src,
 // Host context
 // ===========================================================================
 using namespace usg;
+static_assert(kXcd == kParts, "one spatial part per XCD");
 
 static thread_local std::string g_err;
 
@@ -798,7 +801,8 @@ struct usg_ctx {
   DevState ds{};
   hipStream_t own = nullptr, stream = nullptr;
   std::vector<void *> allocs;
-  int *d_fwd = nullptr, *d_bwd = nullptr;
+  int *d_fwd = nullptr, *d_bwd = nullptr;  // part-grouped wave lists
+  int *d_cell_list = nullptr, *d_face_list = nullptr;
   int *d_wall_faces = nullptr, *d_wall_cells = nullptr;
   double *d_partial = nullptr, *d_norms = nullptr, *d_force = nullptr;
   int nblk = 0;
@@ -866,10 +870,23 @@ static void freestream(usg_ctx *c) {
 
 static int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
+static XSlice slice(const int *off) {
+  XSlice sl;
+  for (int p = 0; p <= kXcd; p++) sl.off[p] = off[p];
+  return sl;
+}
+// grid of a part-split launch: kXcd x (blocks of the largest part)
+static int xgrid(const XSlice &sl) {
+  int mx = 0;
+  for (int p = 0; p < kXcd; p++) mx = std::max(mx, sl.off[p + 1] - sl.off[p]);
+  return kXcd * std::max(1, (mx + kBlock - 1) / kBlock);
+}
+
 template <int FLOW>
 static void launch_residual(usg_ctx *c) {
-  const int nb = c->nblk;
-  hipLaunchKernelGGL(k_grad<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
+  hipLaunchKernelGGL(k_grad<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
+                     cs);
   if (c->timing) {
     if (c->ev_used == c->ev.size()) {
       hipEvent_t a, b;
@@ -879,12 +896,14 @@ static void launch_residual(usg_ctx *c) {
     }
     hipEventRecord(c->ev[c->ev_used].first, c->stream);
   }
-  hipLaunchKernelGGL(k_face<FLOW>, dim3(grid(c->hm.nf)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  hipLaunchKernelGGL(k_face<FLOW>, dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_face_list,
+                     fs);
   if (c->timing) {
     hipEventRecord(c->ev[c->ev_used].second, c->stream);
     c->ev_used++;
   }
-  hipLaunchKernelGGL(k_cell<FLOW>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k);
+  hipLaunchKernelGGL(k_cell<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
+                     cs);
 }
 
 static void launch_residual_any(usg_ctx *c) {
@@ -898,32 +917,17 @@ static void launch_residual_any(usg_ctx *c) {
 template <int FLOW>
 static void launch_sweeps(usg_ctx *c) {
   const HostMesh &h = c->hm;
-  if (h.use_ranges) {
-    const auto &r = h.ranges;
-    const int nw = (int)r.size() - 1;
-    for (int w = 0; w < nw; w++) {
-      int cnt = r[w + 1] - r[w];
-      hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                         (const int *)nullptr, r[w], cnt);
-    }
-    for (int w = nw - 1; w >= 0; w--) {
-      int cnt = r[w + 1] - r[w];
-      hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                         (const int *)nullptr, r[w], cnt);
-    }
-    return;
+  const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
+  for (int w = 0; w < nw; w++) {
+    XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
+    hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                       c->d_fwd, sl);
   }
-  const auto &fo = h.fwd_off;
-  for (size_t w = 0; w + 1 < fo.size(); w++) {
-    int cnt = fo[w + 1] - fo[w];
-    hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       (const int *)c->d_fwd, fo[w], cnt);
-  }
-  const auto &bo = h.bwd_off;
-  for (size_t w = 0; w + 1 < bo.size(); w++) {
-    int cnt = bo[w + 1] - bo[w];
-    hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(grid(cnt)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       (const int *)c->d_bwd, bo[w], cnt);
+  const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
+  for (int w = 0; w < nb; w++) {
+    XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
+    hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                       c->d_bwd, sl);
   }
 }
 
@@ -1051,8 +1055,10 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(upload(c, (double **)&dm.fA, h.fa.data(), nf));
   CK(upload(c, (double **)&dm.fcx, fcx.data(), nf));
   CK(upload(c, (double **)&dm.fcy, fcy.data(), nf));
-  CK(upload(c, &c->d_fwd, h.fwd_cells.data(), nc));
-  CK(upload(c, &c->d_bwd, h.bwd_cells.data(), nc));
+  CK(upload(c, &c->d_fwd, h.fwd_list.data(), nc));
+  CK(upload(c, &c->d_bwd, h.bwd_list.data(), nc));
+  CK(upload(c, &c->d_cell_list, h.cell_list.data(), nc));
+  CK(upload(c, &c->d_face_list, h.face_list.data(), nf));
   std::vector<int> wc(h.wall_faces.size());
   for (size_t i = 0; i < wc.size(); i++) wc[i] = h.flc[h.wall_faces[i]];
   CK(upload(c, &c->d_wall_faces, h.wall_faces.data(), h.wall_faces.size()));
@@ -1275,8 +1281,8 @@ int usg_forces(usg_ctx *c, double *cl, double *cd) {
 int usg_sweep_waves(usg_ctx *c, int *nfwd, int *nbwd) {
   if (!c || !nfwd || !nbwd) return fail(USG_ERR_ARG, "null argument");
   const HostMesh &h = c->hm;
-  *nfwd = h.use_ranges ? (int)h.ranges.size() - 1 : (int)h.fwd_off.size() - 1;
-  *nbwd = h.use_ranges ? (int)h.ranges.size() - 1 : (int)h.bwd_off.size() - 1;
+  *nfwd = (int)h.fwd_poff.size() / (kXcd + 1);
+  *nbwd = (int)h.bwd_poff.size() / (kXcd + 1);
   return USG_OK;
 }
 

@@ -233,12 +233,13 @@ def cell_centres(m: Mesh) -> np.ndarray:
     return acc / cnt[:, None]
 
 
-def colour_hilbert_order(m: Mesh) -> tuple[np.ndarray, int]:
+def colour_hilbert_order(m: Mesh, parts: int = 8) -> tuple[np.ndarray, int]:
     """Permutation ``perm`` (new -> old cell) grouping cells by greedy colour.
 
     Cells are visited in Hilbert order of their centres and given the smallest
     colour not used by an already coloured neighbour; the new numbering sorts
-    by (colour, Hilbert key).  No two cells of one colour are neighbours, so a
+    by (colour, spatial part, Hilbert key), the parts being ``parts`` equal
+    slices of the Hilbert curve.  No two cells of one colour are neighbours, so a
     forward LU-SGS sweep over this numbering can process each colour block as
     one parallel wave (depth = number of colours), while the Hilbert order
     inside each block keeps neighbour gathers local.
@@ -263,18 +264,24 @@ def colour_hilbert_order(m: Mesh) -> tuple[np.ndarray, int]:
             k += 1
         col[cell] = k
     colour = np.array(col, dtype=np.int64)
-    perm = np.lexsort((key, colour))
+    # 8 equal spatial parts along the Hilbert curve (one per XCD of the GPU):
+    # inside a colour block the cells are grouped by part, then Hilbert order
+    rank = np.empty(m.nc, dtype=np.int64)
+    rank[visit] = np.arange(m.nc)
+    part = rank * parts // m.nc
+    perm = np.lexsort((key, part, colour))
     return perm, int(colour.max()) + 1
 
 
-def renumber(m: Mesh, perm: np.ndarray) -> Mesh:
+def renumber(m: Mesh, perm: np.ndarray, part=None) -> Mesh:
     """Apply a cell permutation (new -> old) and rebuild the face order.
 
     Interior faces get ``lC < rC`` (the convention of the shipped meshes,
     flipping node order so the normal still points out of the left cell);
     faces are sorted by (lower cell, upper cell) with boundary faces of a cell
     after its interior faces, so each cell's ascending face list starts with
-    the faces it owns.
+    the faces it owns; with ``part`` (old cell -> spatial part) the faces are
+    first grouped by the part of their left cell.
     """
     inv = np.empty(m.nc, dtype=np.int64)
     inv[perm] = np.arange(m.nc)
@@ -286,15 +293,24 @@ def renumber(m: Mesh, perm: np.ndarray) -> Mesh:
     rc2 = np.where(swap, lc, rc)
     fnode[swap] = fnode[swap][:, ::-1]
     sort_r = np.where(rc2 >= 0, rc2, m.nc + (-rc2))
-    order = np.lexsort((sort_r, lc2))
+    if part is None:
+        order = np.lexsort((sort_r, lc2))
+    else:
+        pl = np.asarray(part)[perm][lc2]
+        order = np.lexsort((sort_r, lc2, pl))
     return Mesh(m.xy.copy(), fnode[order].astype(np.int32), lc2[order].astype(np.int32),
                 rc2[order].astype(np.int32), m.nc, m.name)
 
 
-def prepare(m: Mesh) -> tuple[Mesh, int]:
-    """Colour-blocked Hilbert renumbering; returns (mesh, ncolours)."""
-    perm, ncol = colour_hilbert_order(m)
-    return renumber(m, perm), ncol
+def prepare(m: Mesh, parts: int = 8) -> tuple[Mesh, int]:
+    """Colour-blocked, part-grouped Hilbert renumbering; returns (mesh, ncolours)."""
+    perm, ncol = colour_hilbert_order(m, parts)
+    c = cell_centres(m)
+    key = _hilbert_keys(c[:, 0], c[:, 1])
+    rank = np.empty(m.nc, dtype=np.int64)
+    rank[np.argsort(key, kind="stable")] = np.arange(m.nc)
+    part = rank * parts // m.nc
+    return renumber(m, perm, part), ncol
 
 
 # ---------------------------------------------------------------------------
