@@ -53,7 +53,8 @@ struct DevState {
   double *dt;           // [nc] LU-SGS local time step
   double *dt_rk;        // [nc] RK5 time step, frozen at stage 0
   double *W0;           // [4][nc] RK5 stage origin
-  double *Fc, *Fv, *Ft; // face fluxes: convective [4][nf], viscous [4][nf], SST [2][nf]
+  double *FF;           // face terms [q][nf] (q: flux 0-3, spectral 4, viscous flux 5-8,
+                        // viscous spectral 9, SST 10-11), oriented out of the left cell
   double *D;            // [nc] LU-SGS diagonal
   double *tD;           // [2][nc] SST diagonal
   double *mprc;         // [nc] mu/Pr of the cell (laminar + turbulent) for r_A

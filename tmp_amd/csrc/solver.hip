@@ -6,11 +6,11 @@
 // (faces visited k = 0..nf-1, each adding to its left and right cell) without
 // floating-point atomics.  Kernels per LU-SGS iteration:
 //   k_grad    boundary states, Green-Gauss gradients (+grad T, +grad k/omega),
-//             spectral radii, Venkatakrishnan limiter, SST F1/F2   (per cell)
-//   k_face    limited reconstruction + HLLC, viscous and SST fluxes (per face,
-//             each face once)
-//   k_cell    residual = ordered gather of the face fluxes, SST source, and
-//             the LU-SGS diagonal factors                          (per cell)
+//             Venkatakrishnan limiter, SST F1/F2                   (per cell)
+//   k_face    limited reconstruction + HLLC, viscous and SST fluxes and the
+//             spectral-radius terms, each face once                 (per face)
+//   k_cell    residual and spectral radii = ordered, signed gather of the
+//             face terms, SST source, and the LU-SGS diagonal factors (per cell)
 //   k_sweep   one launch per wave of the forward and the backward sweep; the
 //             backward wave of each cell also writes the updated state into
 //             the second buffer (no separate update pass)
@@ -79,21 +79,9 @@ __device__ __forceinline__ bool get_slot(const DevMesh &m, int s, int c, Slot &o
   return true;
 }
 
-// face eddy viscosity of ComputeTurbulenceViscousCoefficient (SSTKWStrategy.h:755-776)
-__device__ __forceinline__ double face_mut(const Consts &k, const DevState &s, int nb, bool left, double mutc,
-                                           const double *Ue, double n0, double n1) {
-  if (nb >= 0) {
-    double mnb = s.mut[nb];
-    return left ? 0.5 * (mutc + mnb) : 0.5 * (mnb + mutc);
-  }
-  if (nb == kWallD) return 0.0;
-  double Vn = dot2(Ue[2], Ue[3], n0, n1);
-  return Vn > 0.0 ? mutc : k.fsnd_mut;
-}
-
 // ---------------------------------------------------------------------------
-// k_grad: boundary states + gradients + spectral radii + limiter (+ SST F1/F2)
-// Euler2D::updateEdgeValues/solveGradient/solveSpectralRadius (Euler2D.cpp:202-266),
+// k_grad: boundary states + gradients + limiter (+ SST F1/F2)
+// Euler2D::updateEdgeValues/solveGradient (Euler2D.cpp:202-240),
 // ViscousFlow2D.cpp:174-252, Vankatakrishnan_Limiter.hpp:69-200,
 // LaminarFluxStrategy::computeMu (cells), SSTKWStrategy.h:616-722.
 // ---------------------------------------------------------------------------
@@ -111,14 +99,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double gT[2] = {0, 0};
   double tg[4] = {0, 0, 0, 0};
-  double lc = 0, lv = 0;
   double umax[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
   double umin[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
-  double tUc[2] = {0, 0}, mutc = 0, muc = 0;
+  double tUc[2] = {0, 0}, muc = 0;
   if (F::sst) {
     tUc[0] = s.tU[c];
     tUc[1] = s.tU[nc + c];
-    mutc = s.mut[c];
   }
   int bad = 0;
   for (int sl = 0; sl < 4; sl++) {
@@ -148,9 +134,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
         g[2 * j + 1] = g[2 * j + 1] - n1 * (A * Ue[j]);
       }
     }
-    double cs = sqrt(Gamma * Ue[1] / Ue[0]);
-    double Vn = dot2(Ue[2], Ue[3], n0, n1);
-    lc += (fabs(Vn) + cs) * A;
     if (F::visc) {
       double edgeT = Ue[1] / Ue[0];
       if (o.left) {
@@ -160,11 +143,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
         gT[0] = gT[0] - n0 * (A * edgeT);
         gT[1] = gT[1] - n1 * (A * edgeT);
       }
-      double mue = sutherland(k, edgeT);
-      double mpr = mue / k.pr;
-      if (F::sst) mpr = mpr + face_mut(k, s, o.nb, o.left, mutc, Ue, n0, n1) / k.prt;
-      double Sv = smax(4 / (3.0 * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
-      lv += Sv / V;
       if (F::sst) {
         // ComputeFaceTValue (SSTKWStrategy.h:616-659)
         double te[2];
@@ -175,8 +153,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
         } else if (o.nb == kWallD) {
           double dis = m.wdist[c];
           te[0] = 0.0;
-          te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
-        } else if (Vn > 0.0) {
+          te[1] = 10.0 * 6.0 * sutherland(k, edgeT) / (Ue[0] * SST_BETA1 * dis * dis);
+        } else if (dot2(Ue[2], Ue[3], n0, n1) > 0.0) {
           te[0] = tUc[0];
           te[1] = tUc[1];
         } else {
@@ -197,13 +175,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   }
   for (int j = 0; j < 8; j++) g[j] = g[j] / V;
   for (int j = 0; j < 8; j++) s.grad[j * nc + c] = g[j];
-  s.lamc[c] = lc;
   if (F::visc) {
     gT[0] = gT[0] / V;
     gT[1] = gT[1] / V;
     s.gradT[c] = gT[0];
     s.gradT[nc + c] = gT[1];
-    s.lamv[c] = lv;
     double T = Uc[1] / Uc[0];
     muc = sutherland(k, T);
     s.mu[c] = muc;
@@ -291,10 +267,19 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
     boundary_state<F::visc>(k, Rr, UL, n0, n1, Ue);
     boundary_flux(Rr, Ue, n0, n1, A, Fx);
   }
-  for (int j = 0; j < 4; j++) s.Fc[j * nf + f] = Fx[j];
-  if (!F::visc) return;
+  // face record, quantity q at FF[q*nf + f]: flux 0-3, spectral term 4,
+  // viscous flux 5-8, viscous spectral term 9, SST 10-11
+  double *const FF = s.FF + f;
+  for (int j = 0; j < 4; j++) FF[(size_t)j * nf] = Fx[j];
   if (inner)
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
+  {
+    // spectral radius term (Euler2D::solveSpectralRadius, Euler2D.cpp:242-266)
+    double cs = sqrt(Gamma * Ue[1] / Ue[0]);
+    double Vn = dot2(Ue[2], Ue[3], n0, n1);
+    FF[4 * (size_t)nf] = (fabs(Vn) + cs) * A;
+  }
+  if (!F::visc) return;
   double mue = sutherland(k, Ue[1] / Ue[0]);
   double mutf = 0.0;
   if (F::sst) {
@@ -349,10 +334,15 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   double taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
   double qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
   double qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
-  s.Fv[f] = 0.0;
-  s.Fv[nf + f] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
-  s.Fv[2 * nf + f] = (taoxx * n0 + taoxy * n1) * A;
-  s.Fv[3 * nf + f] = (taoxy * n0 + taoyy * n1) * A;
+  double Fv[4];
+  Fv[0] = 0.0;
+  Fv[1] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
+  Fv[2] = (taoxx * n0 + taoxy * n1) * A;
+  Fv[3] = (taoxy * n0 + taoyy * n1) * A;
+  for (int j = 0; j < 4; j++) FF[(size_t)(5 + j) * nf] = Fv[j];
+  // viscous spectral radius term (ViscousFlow2D.cpp:243-249); each cell
+  // divides it by its own volume
+  FF[9 * (size_t)nf] = smax(4 / (3.0 * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
   if (!F::sst) return;
   // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
   const double Vn = dot2(Ue[2], Ue[3], n0, n1);
@@ -412,15 +402,16 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   double Fv1 = txx_w * n0 + tyy_w * n1;
   Fv0 *= A;
   Fv1 *= A;
-  s.Ft[f] = Fct[0] - Fv0;
-  s.Ft[nf + f] = Fct[1] - Fv1;
+  FF[10 * (size_t)nf] = Fct[0] - Fv0;
+  FF[11 * (size_t)nf] = Fct[1] - Fv1;
 }
 
 // ---------------------------------------------------------------------------
 // k_cell: residual of one cell as the reference's scatter order gives it --
 // convective face fluxes in ascending face order, then the viscous pass
 // (R_L -= Fv, R_R += Fv, LaminarFluxStrategy.h:192-210), SST source then
-// faces (SolveTotalTFlux, SSTKWStrategy.h:969-1013, source sign -Q*V) -- and
+// faces (SolveTotalTFlux, SSTKWStrategy.h:969-1013, source sign -Q*V), the
+// spectral radii (Euler2D.cpp:242-266, ViscousFlow2D.cpp:220-252) -- and
 // the per-cell LU-SGS factors: dt (SolveTime, Euler2D.h:67-77 /
 // ViscousFlow2D.h:223-242), D (SolveDiag, TopologyHolderStrategy.cpp:190-215),
 // the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
@@ -439,32 +430,45 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
     if (e == kNoSlot) break;
     fs[ns++] = e;
   }
+  // faces in ascending order, each added with this cell's sign (left +,
+  // right -): the reference's scatter order
   double Rc[4] = {0, 0, 0, 0};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    const int f = e >= 0 ? e : -e - 1;
+    const double *FF = s.FF + (e >= 0 ? e : -e - 1);
     if (e >= 0) {
-      for (int j = 0; j < 4; j++) Rc[j] += s.Fc[j * nf + f];
+      for (int j = 0; j < 4; j++) Rc[j] += FF[(size_t)j * nf];
     } else {
-      for (int j = 0; j < 4; j++) Rc[j] -= s.Fc[j * nf + f];
+      for (int j = 0; j < 4; j++) Rc[j] -= FF[(size_t)j * nf];
     }
   }
   if (F::visc) {
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
-      const int f = e >= 0 ? e : -e - 1;
+      const double *FF = s.FF + (e >= 0 ? e : -e - 1);
       if (e >= 0) {
-        for (int j = 0; j < 4; j++) Rc[j] -= s.Fv[j * nf + f];
+        for (int j = 0; j < 4; j++) Rc[j] -= FF[(size_t)(5 + j) * nf];
       } else {
-        for (int j = 0; j < 4; j++) Rc[j] += s.Fv[j * nf + f];
+        for (int j = 0; j < 4; j++) Rc[j] += FF[(size_t)(5 + j) * nf];
       }
     }
   }
+  const double V = m.vol[c];
+  double lamc = 0, lamv = 0;
+  for (int i = 0; i < ns; i++) {
+    const int e = fs[i];
+    lamc += s.FF[(e >= 0 ? e : -e - 1) + 4 * (size_t)nf];
+  }
+  s.lamc[c] = lamc;
+  if (F::visc) {
+    for (int i = 0; i < ns; i++) {
+      const int e = fs[i];
+      lamv += s.FF[(e >= 0 ? e : -e - 1) + 9 * (size_t)nf] / V;
+    }
+    s.lamv[c] = lamv;
+  }
   store4(s.R, nc, c, Rc);
   const double OMEGAN = 1.5;
-  const double V = m.vol[c];
-  const double lamc = s.lamc[c];
-  const double lamv = F::visc ? s.lamv[c] : 0.0;
   double dtc;
   if (F::sst)
     dtc = k.cfl * V / (lamc + 4.0 * lamv);
@@ -500,13 +504,13 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   double trc[2] = {-(Q0 * V), -(Q1 * V)};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    const int f = e >= 0 ? e : -e - 1;
+    const double *FF = s.FF + (e >= 0 ? e : -e - 1);
     if (e >= 0) {
-      trc[0] += s.Ft[f];
-      trc[1] += s.Ft[nf + f];
+      trc[0] += FF[10 * (size_t)nf];
+      trc[1] += FF[11 * (size_t)nf];
     } else {
-      trc[0] -= s.Ft[f];
-      trc[1] -= s.Ft[nf + f];
+      trc[0] -= FF[10 * (size_t)nf];
+      trc[1] -= FF[11 * (size_t)nf];
     }
   }
   s.tR[c] = trc[0];
@@ -870,6 +874,10 @@ static void freestream(usg_ctx *c) {
 
 static int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
+// quantities per face: flux 4 + spectral 1 (+ viscous flux 4 + viscous
+// spectral 1) (+ SST 2)
+static int slot_q(int flow) { return flow == USG_SST ? 12 : (flow == USG_LAMINAR ? 10 : 5); }
+
 static XSlice slice(const int *off) {
   XSlice sl;
   for (int p = 0; p <= kXcd; p++) sl.off[p] = off[p];
@@ -1078,14 +1086,13 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(dalloc(c, &s.dt, nc));
   CK(dalloc(c, &s.dt_rk, nc));
   CK(dalloc(c, &s.W0, 4 * (size_t)nc));
-  CK(dalloc(c, &s.Fc, 4 * (size_t)nf));
+  CK(dalloc(c, &s.FF, (size_t)slot_q(params->flow) * nf));
   CK(dalloc(c, &s.D, nc));
   CK(dalloc(c, &s.err, 2));
   if (visc) {
     CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
     CK(dalloc(c, &s.lamv, nc));
     CK(dalloc(c, &s.mu, nc));
-    CK(dalloc(c, &s.Fv, 4 * (size_t)nf));
     CK(dalloc(c, &s.mprc, nc));
   }
   if (sst) {
@@ -1101,7 +1108,6 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
     CK(dalloc(c, &s.tR, 2 * (size_t)nc));
     CK(dalloc(c, &s.tdW1, 2 * (size_t)nc));
     CK(dalloc(c, &s.tdW, 2 * (size_t)nc));
-    CK(dalloc(c, &s.Ft, 2 * (size_t)nf));
     CK(dalloc(c, &s.tD, 2 * (size_t)nc));
   }
   CK(dalloc(c, &c->d_partial, c->nblk));
