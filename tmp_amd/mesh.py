@@ -415,7 +415,7 @@ def naca4(t: float, m_camber: float = 0.0, p: float = 0.4, n: int = 256) -> np.n
 
 def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: float = 4.5e-6,
                    rfar: float = 50.0, growth: float = 1.15, seed: int | None = None,
-                   jitter: float = 0.0) -> Mesh:
+                   jitter: float = 0.0, quad_only: bool = False) -> Mesh:
     """Hybrid O-mesh around a 4-digit airfoil (chord 1, far field at ``rfar``).
 
     Layers leave the wall with first spacing ``dy1`` growing by ``growth``
@@ -423,7 +423,8 @@ def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: flo
     circumferential spacing 2*pi*(0.5+eta)/ni); from there the step follows
     that spacing, so the outer cells have aspect ratio ~1 out to the far field.
     The anisotropic wall layers are quadrilaterals, the isotropic region is
-    split into Delaunay triangles.  ``ni`` points around the contour.
+    split into Delaunay triangles (kept as quadrilaterals with
+    ``quad_only``).  ``ni`` points around the contour.
     """
     c = naca4(t, camber, n=ni)
     c = c[:-1] if np.allclose(c[0], c[-1]) else c
@@ -455,7 +456,7 @@ def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: flo
     eta = np.array(eta)
     eta *= rfar / eta[-1]
     nj = len(eta) - 1
-    if nq is None:
+    if nq is None or quad_only:
         nq = nj
     nt = nj - nq
     # marching with smoothed normals: each layer steps along its own normals,
