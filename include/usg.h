@@ -126,9 +126,14 @@ int usg_get_geometry(usg_ctx *ctx, double *vol, double *cc, double *fn, double *
 int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *fn, double *fa,
                    double *fc, double *wdist, int *nfwd, int *nbwd);
 
-/* Kernel timing: when enabled, every launch of the dominant kernel (the cell
- * flux/residual kernel) is bracketed by HIP events on its stream; the total
- * and the count are read (and reset) with usg_kernel_time. */
+/* LU-SGS iterations run as replays of a captured two-iteration hipGraph
+ * (default on); off = eager launches.  Results are identical either way. */
+int usg_use_graphs(usg_ctx *ctx, int on);
+
+/* Kernel timing: when enabled, every launch of the dominant kernel (the face
+ * flux kernel) is bracketed by HIP events on its stream; iterations then run
+ * eagerly (not as graph replays).  The total and the count are read (and
+ * reset) with usg_kernel_time. */
 int usg_timing_enable(usg_ctx *ctx, int on);
 int usg_kernel_time(usg_ctx *ctx, double *total_ms, long *launches);
 

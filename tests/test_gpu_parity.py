@@ -155,3 +155,23 @@ def test_cpp_adapter_driver():
         orc.lusgs_step()
     assert abs(res - orc.res_l2()) <= 1e-12 * orc.res_l2()
     assert np.allclose(u0, orc.U[0], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("flow", [usg.EULER, usg.SST])
+def test_graph_replays_match_eager(flow):
+    """Captured two-iteration graphs replayed with eager work and host
+    synchronisation in between give bit-identical states and residual
+    histories to eager launches, after every call (odd counts included)."""
+    m = airfoil_small()
+    kw = dict(flow=flow, mach=0.15 if flow == usg.SST else 0.5, aoa=3.0, reynolds=6e6)
+    g = usg.Solver(m, **kw)
+    e = usg.Solver(m, **kw)
+    e.graphs(False)
+    for n in (2, 4, 3, 2, 6, 1):
+        hg = g.lusgs(n, history=True)
+        he = e.lusgs(n, history=True)
+        assert np.array_equal(hg, he)
+        assert np.array_equal(g.U, e.U)
+        g.residual()          # eager work between replays
+        e.residual()
+        assert np.array_equal(g.R, e.R)

@@ -722,6 +722,30 @@ __global__ __launch_bounds__(kBlock) void k_norm_final(const double *partial, in
   if (threadIdx.x == 0) *out = sqrt(sh[0]);
 }
 
+// Ring variant of k_norm_final for captured graphs: writes slot (*ctr % cap)
+// and advances the device counter.
+__global__ __launch_bounds__(kBlock) void k_norm_ring(const double *partial, int n, double *ring, int cap,
+                                                      int *ctr) {
+  __shared__ double sh[kBlock];
+  double v = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int i = *ctr;
+    ring[i % cap] = sqrt(sh[0]);
+    *ctr = i + 1;
+  }
+}
+
+__global__ void k_set_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // Wall force: sum over wall faces of (p n A) from the left cell's boundary state.
 __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const int *wall_faces,
                                                    const int *wall_cells, int nw, double *out2) {
@@ -816,7 +840,16 @@ struct usg_ctx {
   size_t ev_used = 0;
   double fs_aoa_rad = 0;
   bool have_residual = false;
+  // captured two-iteration LU-SGS graphs, one per starting state buffer
+  bool use_graph = true;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double *gstart[2] = {nullptr, nullptr};
+  hipStream_t gstream[2] = {nullptr, nullptr};
+  double *d_ring = nullptr;
+  int *d_ctr = nullptr;
 };
+
+constexpr int kRing = 4096;
 
 template <typename T>
 static int dalloc(usg_ctx *c, T **p, size_t n) {
@@ -891,35 +924,40 @@ static int xgrid(const XSlice &sl) {
 }
 
 template <int FLOW>
-static void launch_residual(usg_ctx *c) {
+static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
   hipLaunchKernelGGL(k_grad<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
                      cs);
-  if (c->timing) {
-    if (c->ev_used == c->ev.size()) {
-      hipEvent_t a, b;
-      hipEventCreate(&a);
-      hipEventCreate(&b);
-      c->ev.push_back({a, b});
-    }
-    hipEventRecord(c->ev[c->ev_used].first, c->stream);
-  }
+  if (ea) hipEventRecord(ea, c->stream);
   hipLaunchKernelGGL(k_face<FLOW>, dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_face_list,
                      fs);
-  if (c->timing) {
-    hipEventRecord(c->ev[c->ev_used].second, c->stream);
-    c->ev_used++;
-  }
+  if (eb) hipEventRecord(eb, c->stream);
   hipLaunchKernelGGL(k_cell<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
                      cs);
 }
 
-static void launch_residual_any(usg_ctx *c) {
+static void launch_residual_events(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   switch (c->prm.flow) {
-    case USG_EULER: launch_residual<USG_EULER>(c); break;
-    case USG_LAMINAR: launch_residual<USG_LAMINAR>(c); break;
-    default: launch_residual<USG_SST>(c); break;
+    case USG_EULER: launch_residual<USG_EULER>(c, ea, eb); break;
+    case USG_LAMINAR: launch_residual<USG_LAMINAR>(c, ea, eb); break;
+    default: launch_residual<USG_SST>(c, ea, eb); break;
   }
+}
+
+// Eager residual; in timing mode the flux kernel is bracketed by a pooled event pair.
+static void launch_residual_any(usg_ctx *c) {
+  if (!c->timing) {
+    launch_residual_events(c, nullptr, nullptr);
+    return;
+  }
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t a, b;
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    c->ev.push_back({a, b});
+  }
+  launch_residual_events(c, c->ev[c->ev_used].first, c->ev[c->ev_used].second);
+  c->ev_used++;
 }
 
 template <int FLOW>
@@ -943,6 +981,8 @@ static void launch_norm(usg_ctx *c, double *dst) {
   hipLaunchKernelGGL(k_norm_partial, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_partial);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, c->stream, c->d_partial, c->nblk, dst);
 }
+
+static double *stbuf(usg_ctx *c) { return c->ds.U; }
 
 static void swap_state(usg_ctx *c) {
   std::swap(c->ds.U, c->ds.Un);
@@ -979,6 +1019,8 @@ static int check_err(usg_ctx *c) {
   return USG_OK;
 }
 
+static void drop_graphs(usg_ctx *c);
+
 extern "C" {
 
 const char *usg_last_error(void) { return g_err.c_str(); }
@@ -986,6 +1028,7 @@ const char *usg_last_error(void) { return g_err.c_str(); }
 int usg_ctx_destroy(usg_ctx *c) {
   if (!c) return USG_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   for (auto &e : c->ev) {
     hipEventDestroy(e.first);
     hipEventDestroy(e.second);
@@ -1001,6 +1044,7 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   *out = nullptr;
   if (params->flow < USG_EULER || params->flow > USG_SST) return fail(USG_ERR_ARG, "Undefined Flow Type");
   if (params->precision != USG_FP64 && params->precision != USG_MIXED) return fail(USG_ERR_ARG, "bad precision");
+  if (params->precision == USG_MIXED) return fail(USG_ERR_ARG, "USG_MIXED is not built into this library");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
   usg_ctx *c = new usg_ctx();
@@ -1113,6 +1157,8 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(dalloc(c, &c->d_partial, c->nblk));
   CK(dalloc(c, &c->d_force, 2));
   CK(ensure_norms(c, 16));
+  CK(dalloc(c, &c->d_ring, kRing));
+  CK(dalloc(c, &c->d_ctr, 1));
 #undef CK
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     usg_ctx_destroy(c);
@@ -1175,25 +1221,118 @@ int usg_residual(usg_ctx *c) {
   return check_err(c);
 }
 
+static void launch_sweeps_any(usg_ctx *c) {
+  switch (c->prm.flow) {
+    case USG_EULER: launch_sweeps<USG_EULER>(c); break;
+    case USG_LAMINAR: launch_sweeps<USG_LAMINAR>(c); break;
+    default: launch_sweeps<USG_SST>(c); break;
+  }
+}
+
+static void launch_norm_ring(usg_ctx *c) {
+  hipLaunchKernelGGL(k_norm_partial, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_partial);
+  hipLaunchKernelGGL(k_norm_ring, dim3(1), dim3(kBlock), 0, c->stream, c->d_partial, c->nblk, c->d_ring, kRing,
+                     c->d_ctr);
+}
+
+// One LU-SGS iteration (LUSGSIntegrator::UpdateSerial): residual, norm into
+// the ring, sweeps (the backward sweep writes the update), buffer swap.
+static void one_iteration(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
+  launch_residual_events(c, ea, eb);
+  launch_norm_ring(c);
+  launch_sweeps_any(c);
+  swap_state(c);
+}
+
+static void drop_graphs(usg_ctx *c) {
+  for (int i = 0; i < 2; i++) {
+    if (c->gexec[i]) hipGraphExecDestroy(c->gexec[i]);
+    c->gexec[i] = nullptr;
+    c->gstart[i] = nullptr;
+  }
+}
+
+// Captures two iterations (a -> b -> a) into an executable graph of kernel
+// launches only (timing mode runs eagerly: events recorded inside a graph
+// are not readable).  Capture executes nothing, so the state pointers end
+// where they started.
+static bool build_graph(usg_ctx *c, int slot) {
+  double *start = stbuf(c);
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
+  one_iteration(c, nullptr, nullptr);
+  one_iteration(c, nullptr, nullptr);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess || !g) {
+    (void)hipGetLastError();
+    return false;
+  }
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  c->gexec[slot] = ex;
+  c->gstart[slot] = start;
+  c->gstream[slot] = c->stream;
+  return stbuf(c) == start;
+}
+
+// Both graphs (one per state buffer the pair starts from) are built together,
+// the second with the state pointers swapped on the host only, so neither is
+// ever instantiated inside a caller's timed loop after the first call.
+static bool ensure_graphs(usg_ctx *c) {
+  if (c->gexec[0] && c->gexec[1] && c->gstream[0] == c->stream && c->gstream[1] == c->stream) return true;
+  drop_graphs(c);
+  if (!build_graph(c, 0)) return false;
+  swap_state(c);
+  const bool ok = build_graph(c, 1);
+  swap_state(c);
+  return ok;
+}
+
 int usg_lusgs_step(usg_ctx *c, int n, double *res_hist) {
   if (!c || n < 0) return fail(USG_ERR_ARG, "bad argument");
-  if (res_hist) {
-    int rc = ensure_norms(c, n);
-    if (rc) return rc;
-  }
-  for (int it = 0; it < n; it++) {
-    launch_residual_any(c);
-    if (res_hist) launch_norm(c, c->d_norms + it);
-    switch (c->prm.flow) {
-      case USG_EULER: launch_sweeps<USG_EULER>(c); break;
-      case USG_LAMINAR: launch_sweeps<USG_LAMINAR>(c); break;
-      default: launch_sweeps<USG_SST>(c); break;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->d_ctr, 0);
+  int done = 0;
+  while (done < n) {
+    const int chunk = std::min(n - done, kRing);
+    int i = 0;
+    while (i < chunk) {
+      if (c->use_graph && !c->timing && chunk - i >= 2) {
+        if (!ensure_graphs(c)) {
+          c->use_graph = false;
+          drop_graphs(c);
+          continue;
+        }
+        const int slot = c->gstart[0] == stbuf(c) ? 0 : 1;
+        HIPCHK(hipGraphLaunch(c->gexec[slot], c->stream));
+        i += 2;
+      } else {
+        hipEvent_t ea = nullptr, eb = nullptr;
+        if (c->timing) {
+          if (c->ev_used == c->ev.size()) {
+            hipEvent_t a = nullptr, b = nullptr;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            c->ev.push_back({a, b});
+          }
+          ea = c->ev[c->ev_used].first;
+          eb = c->ev[c->ev_used].second;
+          c->ev_used++;
+        }
+        one_iteration(c, ea, eb);
+        i += 1;
+      }
     }
-    swap_state(c);
+    HIPCHK(hipGetLastError());
+    if (res_hist)
+      HIPCHK(hipMemcpyAsync(res_hist + done, c->d_ring, chunk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->d_ctr, 0);
+    done += chunk;
   }
-  HIPCHK(hipGetLastError());
-  if (res_hist && n > 0)
-    HIPCHK(hipMemcpyAsync(res_hist, c->d_norms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   c->have_residual = false;
   return check_err(c);
 }
@@ -1318,6 +1457,19 @@ int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *f
   if (wdist) std::memcpy(wdist, h.wdist.data(), h.wdist.size() * sizeof(double));
   if (nfwd) *nfwd = (int)h.fwd_off.size() - 1;
   if (nbwd) *nbwd = (int)h.bwd_off.size() - 1;
+  return USG_OK;
+}
+
+int usg_use_graphs(usg_ctx *c, int on) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graphs(c);
+  else if (!ensure_graphs(c)) {
+    c->use_graph = false;
+    drop_graphs(c);
+    return fail(USG_ERR_HIP, "hipGraph capture of the LU-SGS iteration pair failed");
+  }
   return USG_OK;
 }
 

@@ -2,6 +2,13 @@
 kernel call count, total and average duration, share of GPU time.
 
     python -m tmp_amd.profsummary <results.db> [--title T] [--command C]
+    python -m tmp_amd.profsummary --pmc <results.db>
+    python -m tmp_amd.profsummary --traffic KERNEL --cells NC --copy-bytes B \
+        <fetch.db> <write.db> <copy_fetch.db> <copy_write.db>
+
+The traffic mode turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of one
+kernel into HBM bytes per launch, each counter scaled by the factor that makes
+it read B bytes for the plain copy kernel (which moves B bytes each way).
 """
 from __future__ import annotations
 
@@ -28,13 +35,55 @@ def counters(db: str) -> dict:
     return {k: {cn: sum(dv.values()) / len(dv) for cn, dv in cs.items()} for k, cs in acc.items()}
 
 
+def _one(db: str, kernel: str, counter: str) -> float:
+    cs = counters(db)
+    hits = [v[counter] for k, v in cs.items() if kernel in k and counter in v]
+    if len(hits) != 1:
+        raise SystemExit(f"{db}: {len(hits)} kernels match {kernel!r} with {counter}")
+    return hits[0]
+
+
+def traffic(kernel: str, cells: int, copy_bytes: int, dbs: list, command: str) -> None:
+    fetch_db, write_db, cfetch_db, cwrite_db = dbs
+    f_kib = _one(fetch_db, kernel, "FETCH_SIZE")
+    w_kib = _one(write_db, kernel, "WRITE_SIZE")
+    cf_kib = _one(cfetch_db, "k_copy", "FETCH_SIZE")
+    cw_kib = _one(cwrite_db, "k_copy", "WRITE_SIZE")
+    sf = copy_bytes / (cf_kib * 1024.0)
+    sw = copy_bytes / (cw_kib * 1024.0)
+    tb = f_kib * 1024.0 * sf + w_kib * 1024.0 * sw
+    print(f"# HBM traffic of {kernel} per launch\n")
+    if command:
+        print(f"Commands: {command}\n")
+    print("| quantity | value |")
+    print("|---|---:|")
+    print(f"| {kernel} FETCH_SIZE (KiB/dispatch) | {f_kib:.1f} |")
+    print(f"| {kernel} WRITE_SIZE (KiB/dispatch) | {w_kib:.1f} |")
+    print(f"| k_copy FETCH_SIZE (KiB/dispatch, {copy_bytes} B copied) | {cf_kib:.1f} |")
+    print(f"| k_copy WRITE_SIZE (KiB/dispatch) | {cw_kib:.1f} |")
+    print(f"| FETCH scale factor | {sf:.4f} |")
+    print(f"| WRITE scale factor | {sw:.4f} |")
+    print(f"| read bytes per launch | {f_kib * 1024.0 * sf:.4g} |")
+    print(f"| written bytes per launch | {w_kib * 1024.0 * sw:.4g} |")
+    print()
+    print(f"cells: {cells}")
+    print(f"traffic_bytes_per_launch: {tb:.6g}")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("db")
+    ap.add_argument("db", nargs="+")
+    ap.add_argument("--traffic", metavar="KERNEL")
+    ap.add_argument("--cells", type=int, default=0)
+    ap.add_argument("--copy-bytes", type=int, default=0)
     ap.add_argument("--title", default="rocprofv3 kernel summary")
     ap.add_argument("--command", default="")
     ap.add_argument("--pmc", action="store_true", help="the db holds --pmc counters")
     a = ap.parse_args()
+    if a.traffic:
+        traffic(a.traffic, a.cells, a.copy_bytes, a.db, a.command)
+        return
+    a.db = a.db[0]
     if a.pmc:
         print(f"# {a.title}\n")
         if a.command:

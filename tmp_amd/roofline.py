@@ -4,13 +4,15 @@ Each count assumes every input value is read from HBM once per launch and
 every output written once (ideal caching of neighbour gathers), FP64 = 8 B,
 int32 = 4 B.  ``r`` is faces per cell (nf/nc) of the mesh.
 
-Dominant kernel ``k_flux`` (residual of one cell), per cell:
-  * own-cell inputs:  U 32, grad 64, phi 32, cell centre 16 (all flows)
-                      + gradT 16 (viscous)
-                      + tU 16, tgrad 32, F1 8, mut 8, vol 8, wdist 8 (SST)
-  * face slots: 4 x (face id + neighbour id) = 32
-  * face data: normal 16 + length 8 + midpoint 16 = 40 per face
-  * outputs: R 32 (+ turbulence residual 16 for SST)
+Dominant kernel ``k_face`` (every flux of one face), per launch over the
+whole mesh = nc x cell term + nf x face term:
+  * cell inputs (each cell's values read once): U 32, grad 64, phi 32,
+    cell centre 16 (all flows) + gradT 16 (viscous) + mut 8, tU 16, F1 8,
+    tgrad 32 (SST)                      -> Euler 144, laminar 160, SST 224
+  * face inputs: left/right cell 8, normal 16, length 8, midpoint 16 = 48
+  * face outputs: flux 4 + spectral term 1 (Euler), + viscous flux 4 +
+    viscous spectral term 1 (laminar), + SST flux 2: 40 / 80 / 96
+  so per cell: Euler 144 + 88 r, laminar 160 + 128 r, SST 224 + 144 r.
 
 Whole LU-SGS iteration (the cell-iteration model of the reference path's
 six inherent stages -- gradient, limiter, flux, forward sweep, backward
@@ -20,14 +22,14 @@ sweep, update): Euler 992 + 200 r, laminar 1064 + 200 r, SST 1536 + 200 r.
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def flux_kernel_bytes_per_cell(flow: int, r: float) -> float:
-    own = 32 + 64 + 32 + 16
+def face_kernel_bytes_per_cell(flow: int, r: float) -> float:
+    cell = 32 + 64 + 32 + 16
     if flow >= 1:
-        own += 16
+        cell += 16
     if flow == 2:
-        own += 16 + 32 + 8 + 8 + 8 + 8
-    out = 32 + (16 if flow == 2 else 0)
-    return own + 32 + 40.0 * r + out
+        cell += 8 + 16 + 8 + 32
+    face_out = {0: 40, 1: 80, 2: 96}[flow]
+    return cell + (48 + face_out) * r
 
 
 def iteration_bytes_per_cell(flow: int, r: float) -> float:

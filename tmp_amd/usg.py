@@ -51,7 +51,7 @@ class _Params(C.Structure):
 EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "usg_init_freestream",
            "usg_set_prim", "usg_residual", "usg_lusgs_step", "usg_rk5_step", "usg_get_field", "usg_res_l2",
            "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
-           "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_last_error"]
+           "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error"]
 
 _lib = None
 
@@ -83,6 +83,7 @@ def lib():
         L.usg_kernel_time.argtypes = [vp, _dp, C.POINTER(C.c_long)]
         L.usg_stencil_apply.argtypes = [vp, C.c_int, C.c_int, _ip, _ip, _ip, _dp, vp]
         L.usg_copy_kernel.argtypes = [vp, vp, C.c_long, vp]
+        L.usg_use_graphs.argtypes = [vp, C.c_int]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -201,6 +202,9 @@ class Solver:
 
     def sync(self):
         _check(lib().usg_sync(self._h))
+
+    def graphs(self, on: bool):
+        _check(lib().usg_use_graphs(self._h, int(on)))
 
     def timing(self, on: bool):
         _check(lib().usg_timing_enable(self._h, int(on)))
