@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--workload", default="sst1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed: FP32 gradients/limiter/fluxes, FP64 state, accumulation, diagonal and norms")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (traffic-counter calibration)")
     a = ap.parse_args()
@@ -119,7 +121,8 @@ def main():
     m, ncol = build_mesh(kind)
     print(f"# mesh {m.name}: {m.nc} cells, {m.nf} faces, {ncol} colours, built in {time.perf_counter() - t:.1f}s",
           file=sys.stderr, flush=True)
-    solver = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re)
+    mixed = a.precision == "mixed"
+    solver = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re, precision=usg.MIXED if mixed else usg.FP64)
     nfw, nbw = solver.waves()
     if a.warmup:
         solver.lusgs(a.warmup)
@@ -139,7 +142,7 @@ def main():
     solver.timing(False)
     value = m.nc * a.steps / elapsed
     r = m.nf / m.nc
-    kbytes = roofline.face_kernel_bytes_per_cell(flow, r) * m.nc
+    kbytes = roofline.face_kernel_bytes_per_cell(flow, r, mixed) * m.nc
     kavg_s = ktime_ms / 1e3 / max(klaunch, 1)
     achieved = kbytes / kavg_s / 1e9
     it_frac = roofline.iteration_bytes_per_cell(flow, r) * value / 1e9 / roofline.HBM_PEAK_GBS
@@ -157,14 +160,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic: generated mesh, free-stream cold start",
-        "config": {"workload": label, "cells": m.nc, "faces": m.nf, "colours": ncol,
+        "config": {"workload": label.replace("FP64", "mixed FP32/FP64") if mixed else label, "cells": m.nc, "faces": m.nf, "colours": ncol,
                    "lusgs_waves": [nfw, nbw], "mach": mach, "aoa_deg": aoa, "reynolds": re,
                    "integrator": "LU-SGS CFL 400", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_face", "achieved": achieved,
                      "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / roofline.HBM_PEAK_GBS,
-                     "traffic": measured_traffic(a.workload, m.nc), "iteration_frac": it_frac},
+                     "traffic": None if mixed else measured_traffic(a.workload, m.nc),
+                     "iteration_frac": it_frac},
         "cpu_baseline": None,
     }
     if not a.no_cpu:

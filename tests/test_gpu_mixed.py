@@ -1,0 +1,85 @@
+"""Mixed precision (USG_MIXED: FP32 gradients, limiter and face fluxes; FP64
+state, residual accumulation, LU-SGS diagonal, sweeps and norms) against the
+FP64 path on the GPU, which is itself pinned to the oracle.
+
+Bars, per field, max |mixed - fp64| / max |fp64|:
+  * one residual evaluation on a perturbed state: R <= 2e-4, grad/phi <= 1e-5
+    (FP32 rounding of O(1) face fluxes that cancel to O(1e-2) residuals);
+  * after LU-SGS iterations from the free stream: U <= 1e-4 per variable
+    (1e-3 for the SST transient, whose F1 switch and limiter amplify
+    rounding differences), residual history <= 1e-2 relative, C_L / C_D
+    within 1e-4 absolute.
+The measured errors are printed; the tables in the design notes come from them.
+"""
+import numpy as np
+import pytest
+
+from tmp_amd import usg
+
+from .common import airfoil_small, bump, perturbed, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(m, flow, mach, aoa, re):
+    kw = dict(flow=flow, mach=mach, aoa=aoa, reynolds=re)
+    return usg.Solver(m, precision=usg.FP64, **kw), usg.Solver(m, precision=usg.MIXED, **kw)
+
+
+CASES = [
+    ("bump-euler", bump, usg.EULER, 0.5, 0.0, 1e6),
+    ("airfoil-laminar", airfoil_small, usg.LAMINAR, 0.5, 2.0, 5000.0),
+    ("airfoil-sst", airfoil_small, usg.SST, 0.15, 4.0, 6e6),
+]
+
+
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", CASES, ids=[c[0] for c in CASES])
+def test_mixed_residual_error(name, mk, flow, mach, aoa, re):
+    m = mk()
+    a, b = _pair(m, flow, mach, aoa, re)
+    geom = usg.mesh_geometry(m)
+    fs = a.U[0].tolist()
+    U = perturbed(m, geom["cc"], fs, seed=3)
+    a.set_prim(U)
+    b.set_prim(U)
+    a.residual()
+    b.residual()
+    fields = {"R": usg.F_RES, "grad": usg.F_GRAD, "phi": usg.F_PHI, "lamc": usg.F_LAMC}
+    if flow != usg.EULER:
+        fields["lamv"] = usg.F_LAMV
+    if flow == usg.SST:
+        fields["tR"] = usg.F_TRES
+    bars = {"R": 2e-4, "grad": 1e-5, "phi": 1e-5, "lamc": 1e-5, "lamv": 1e-5, "tR": 2e-3}
+    for k, fid in fields.items():
+        e = np.max(relerr(b.field(fid), a.field(fid)))
+        print(f"{name} mixed {k}: {e:.3e}")
+        assert np.all(np.isfinite(b.field(fid)))
+        assert e <= bars[k], (k, e)
+
+
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re,n", [
+    ("bump-euler", bump, usg.EULER, 0.5, 0.0, 1e6, 300),
+    ("airfoil-laminar", airfoil_small, usg.LAMINAR, 0.5, 2.0, 5000.0, 300),
+    ("airfoil-sst", airfoil_small, usg.SST, 0.15, 4.0, 6e6, 100),
+], ids=["bump-euler", "airfoil-laminar", "airfoil-sst"])
+def test_mixed_trajectory_error(name, mk, flow, mach, aoa, re, n):
+    m = mk()
+    a, b = _pair(m, flow, mach, aoa, re)
+    ha = a.lusgs(n, history=True)
+    hb = b.lusgs(n, history=True)
+    eU = relerr(b.U, a.U)
+    eh = np.max(np.abs(hb - ha) / ha)
+    print(f"{name} mixed after {n}: U per var {np.array2string(eU, precision=3)}; history {eh:.3e}; "
+          f"res {ha[-1] / ha[0]:.3e}")
+    assert np.all(np.isfinite(b.U))
+    assert np.max(eU) <= (1e-3 if flow == usg.SST else 1e-4)
+    assert eh <= 1e-2
+    if "airfoil" in name:
+        cla, cda = a.forces()
+        clb, cdb = b.forces()
+        print(f"{name} CL {cla:.8f} vs {clb:.8f}  CD {cda:.8f} vs {cdb:.8f}")
+        assert abs(clb - cla) <= 1e-4 and abs(cdb - cda) <= 1e-4
+    if flow == usg.SST:
+        et = relerr(b.field(usg.F_TURB), a.field(usg.F_TURB))
+        print(f"{name} mixed (k, omega) {np.array2string(et, precision=3)}")
+        assert np.max(et) <= 1e-3

@@ -5,6 +5,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace usg {
 
 constexpr int kNoSlot = -2147483647 - 1;  // empty face slot of a triangle
@@ -37,6 +39,11 @@ struct DevMesh {
   const int *cf, *cnb;
   const int *flc, *frc;  // face left cell, right cell or kWallD / kFarD
   const double *fnx, *fny, *fA, *fcx, *fcy;
+  // FP32 copies for the mixed-precision kernels (null in FP64 mode):
+  // normal, length, and the offsets fdx[6][nf] = (x_f - x_L, y_f - y_L,
+  // x_f - x_R, y_f - y_R, x_R - x_L, y_R - y_L), differenced in FP64 on the
+  // host so that FP32 keeps their relative precision near the wall
+  const float *fnxf, *fnyf, *fAf, *fdx;
 };
 
 // State, structure-of-arrays ([var*nc + cell]).
@@ -64,7 +71,26 @@ struct DevState {
   double *F1, *F2, *mut, *mutn; // [nc]
   double *tR, *tdW1, *tdW;      // [2][nc]
   int *err;                     // [2] invariant violation counters
+  // FP32 storage of the mixed-precision mode (null in FP64 mode, where the
+  // double arrays above hold these quantities)
+  float *gradf, *phif, *gradTf, *tgradf, *F1f, *F2f, *FFf;
 };
+
+// The array holding a quantity in compute type T.
+template <typename T>
+__host__ __device__ __forceinline__ T *pick(double *d, float *f) {
+  if constexpr (std::is_same<T, double>::value)
+    return d;
+  else
+    return f;
+}
+template <typename T>
+__host__ __device__ __forceinline__ const T *pickc(const double *d, const float *f) {
+  if constexpr (std::is_same<T, double>::value)
+    return d;
+  else
+    return f;
+}
 
 struct Consts {
   double fs[4];        // fs_primVar (rho, p, u, v) non-dimensional
@@ -80,14 +106,17 @@ struct Consts {
 
 #define USG_GAMMA 1.4
 
-__device__ __forceinline__ double dot2(double a0, double a1, double b0, double b1) {
-  double r = 0;
+template <typename T>
+__device__ __forceinline__ T dot2(T a0, T a1, T b0, T b1) {
+  T r = 0;
   r += a0 * b0;
   r += a1 * b1;
   return r;
 }
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+template <typename T>
+__device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 
 // Far-field Riemann-invariant state, BoundaryStrategy.h:262-328 (inviscid) and
 // :120-187 (viscous; identical arithmetic).
@@ -152,12 +181,14 @@ __device__ __forceinline__ bool boundary_state(const Consts &k, int bc, const do
 }
 
 // Sutherland viscosity of LaminarFluxStrategy::computeMu (LaminarFluxStrategy.h:76-96)
-__device__ __forceinline__ double sutherland(const Consts &k, double T) {
-  return (1.0 + k.cc) / (T + k.cc) * pow(T, 1.5) * k.sqrt_gamma * k.mach / k.re;
+template <typename T>
+__device__ __forceinline__ T sutherland(const Consts &k, T Tc) {
+  return (T(1.0) + T(k.cc)) / (Tc + T(k.cc)) * pow(Tc, T(1.5)) * T(k.sqrt_gamma) * T(k.mach) / T(k.re);
 }
 
 // Venkatakrishnan limiter value for one face (Vankatakrishnan_Limiter.hpp:128-139)
-__device__ __forceinline__ double venkat(double d1max, double d1min, double delta2, double eps2) {
+template <typename T>
+__device__ __forceinline__ T venkat(T d1max, T d1min, T delta2, T eps2) {
   if (delta2 > 0)
     return (d1max * d1max + eps2 + 2 * delta2 * d1max) /
            (d1max * d1max + 2 * delta2 * delta2 + d1max * delta2 + eps2);
@@ -167,61 +198,69 @@ __device__ __forceinline__ double venkat(double d1max, double d1min, double delt
   return 1;
 }
 
+// The p* invariant of the interior HLLC flux (HLLCFluxStrategy.cpp:77-86):
+// |p*_L - p*_R| < 1e-10 in FP64.  Single precision cannot meet an absolute
+// 1e-10, so the FP32 flux of the mixed-precision mode checks relative to p*.
+__device__ __forceinline__ bool pstar_ok(double a, double b) { return fabs(a - b) < 1e-10; }
+__device__ __forceinline__ bool pstar_ok(float a, float b) {
+  return fabsf(a - b) < 1.0e-4f * (fabsf(a) + fabsf(b));
+}
+
 // Interior HLLC flux (HLLCFluxStrategy.cpp:155-248).  Returns the p* check.
-__device__ __forceinline__ bool hllc(const double *UL, const double *UR, double n0, double n1, double area,
-                                     double *F) {
-  const double Gamma = USG_GAMMA;
-  double density_left = UL[0], pressure_left = UL[1];
-  double vl0 = UL[2], vl1 = UL[3];
-  double Vn_left = dot2(vl0, vl1, n0, n1);
-  double soundspeed_left = sqrt(Gamma * pressure_left / density_left);
-  double vl2 = dot2(vl0, vl1, vl0, vl1);
-  double enthalpy_left = pressure_left * Gamma / ((Gamma - 1) * density_left) + 0.5 * vl2;
-  double density_righ = UR[0], pressure_righ = UR[1];
-  double vr0 = UR[2], vr1 = UR[3];
-  double Vn_righ = dot2(vr0, vr1, n0, n1);
-  double soundspeed_righ = sqrt(Gamma * pressure_righ / density_righ);
-  double vr2 = dot2(vr0, vr1, vr0, vr1);
-  double enthalpy_righ = pressure_righ * Gamma / ((Gamma - 1) * density_righ) + 0.5 * vr2;
-  double efficient_left = sqrt(density_left) / (sqrt(density_left) + sqrt(density_righ));
-  double efficient_righ = 1 - efficient_left;
-  double enthalpy_tilda = efficient_left * enthalpy_left + efficient_righ * enthalpy_righ;
-  double Vn_tilda = efficient_left * Vn_left + efficient_righ * Vn_righ;
-  double soundspeed_tilda = sqrt((Gamma - 1) * (enthalpy_tilda - 0.5 * pow(Vn_tilda, 2.0)));
-  double Sl = smin(Vn_left - soundspeed_left, Vn_tilda - soundspeed_tilda);
-  double Sr = smax(Vn_tilda + soundspeed_tilda, Vn_righ + soundspeed_righ);
-  double Sm = (density_righ * Vn_righ * (Sr - Vn_righ) - density_left * Vn_left * (Sl - Vn_left) + pressure_left -
+template <typename T>
+__device__ __forceinline__ bool hllc(const T *UL, const T *UR, T n0, T n1, T area, T *F) {
+  const T Gamma = T(USG_GAMMA);
+  T density_left = UL[0], pressure_left = UL[1];
+  T vl0 = UL[2], vl1 = UL[3];
+  T Vn_left = dot2(vl0, vl1, n0, n1);
+  T soundspeed_left = sqrt(Gamma * pressure_left / density_left);
+  T vl2 = dot2(vl0, vl1, vl0, vl1);
+  T enthalpy_left = pressure_left * Gamma / ((Gamma - 1) * density_left) + T(0.5) * vl2;
+  T density_righ = UR[0], pressure_righ = UR[1];
+  T vr0 = UR[2], vr1 = UR[3];
+  T Vn_righ = dot2(vr0, vr1, n0, n1);
+  T soundspeed_righ = sqrt(Gamma * pressure_righ / density_righ);
+  T vr2 = dot2(vr0, vr1, vr0, vr1);
+  T enthalpy_righ = pressure_righ * Gamma / ((Gamma - 1) * density_righ) + T(0.5) * vr2;
+  T efficient_left = sqrt(density_left) / (sqrt(density_left) + sqrt(density_righ));
+  T efficient_righ = 1 - efficient_left;
+  T enthalpy_tilda = efficient_left * enthalpy_left + efficient_righ * enthalpy_righ;
+  T Vn_tilda = efficient_left * Vn_left + efficient_righ * Vn_righ;
+  T soundspeed_tilda = sqrt((Gamma - 1) * (enthalpy_tilda - T(0.5) * pow(Vn_tilda, T(2.0))));
+  T Sl = smin(Vn_left - soundspeed_left, Vn_tilda - soundspeed_tilda);
+  T Sr = smax(Vn_tilda + soundspeed_tilda, Vn_righ + soundspeed_righ);
+  T Sm = (density_righ * Vn_righ * (Sr - Vn_righ) - density_left * Vn_left * (Sl - Vn_left) + pressure_left -
                pressure_righ) /
               (density_righ * (Sr - Vn_righ) - density_left * (Sl - Vn_left));
-  double p_star = density_left * (Vn_left - Sl) * (Vn_left - Sm) + pressure_left;
-  double p_star_scratch = density_righ * (Vn_righ - Sr) * (Vn_righ - Sm) + pressure_righ;
-  bool ok = fabs(p_star_scratch - p_star) < 1e-10;
-  if (Sl > 0.0) {
-    double rhoE = pressure_left / (Gamma - 1) + 0.5 * density_left * vl2;
+  T p_star = density_left * (Vn_left - Sl) * (Vn_left - Sm) + pressure_left;
+  T p_star_scratch = density_righ * (Vn_righ - Sr) * (Vn_righ - Sm) + pressure_righ;
+  bool ok = pstar_ok(p_star_scratch, p_star);
+  if (Sl > T(0.0)) {
+    T rhoE = pressure_left / (Gamma - 1) + T(0.5) * density_left * vl2;
     F[0] = Vn_left * density_left * area;
     F[1] = Vn_left * (rhoE + pressure_left) * area;
     F[2] = (Vn_left * density_left * vl0 + pressure_left * n0) * area;
     F[3] = (Vn_left * density_left * vl1 + pressure_left * n1) * area;
-  } else if (Sr < 0.0) {
-    double rhoE = pressure_righ / (Gamma - 1) + 0.5 * density_righ * vr2;
+  } else if (Sr < T(0.0)) {
+    T rhoE = pressure_righ / (Gamma - 1) + T(0.5) * density_righ * vr2;
     F[0] = Vn_righ * density_righ * area;
     F[1] = Vn_righ * (rhoE + pressure_righ) * area;
     F[2] = (Vn_righ * density_righ * vr0 + pressure_righ * n0) * area;
     F[3] = (Vn_righ * density_righ * vr1 + pressure_righ * n1) * area;
   } else {
-    double SRef, Vn_ref, density_ref, pressure_ref, v0, v1, vv;
-    if (Sm < 0.0) {
+    T SRef, Vn_ref, density_ref, pressure_ref, v0, v1, vv;
+    if (Sm < T(0.0)) {
       SRef = Sr; Vn_ref = Vn_righ; density_ref = density_righ; pressure_ref = pressure_righ;
       v0 = vr0; v1 = vr1; vv = vr2;
     } else {
       SRef = Sl; Vn_ref = Vn_left; density_ref = density_left; pressure_ref = pressure_left;
       v0 = vl0; v1 = vl1; vv = vl2;
     }
-    double rhoE_ref = pressure_ref / (Gamma - 1) + 0.5 * density_ref * vv;
-    double f0 = (SRef - Vn_ref) * density_ref / (SRef - Sm);
-    double f1 = ((SRef - Vn_ref) * rhoE_ref - pressure_ref * Vn_ref + p_star * Sm) / (SRef - Sm);
-    double f2 = ((SRef - Vn_ref) * density_ref * v0 + (p_star - pressure_ref) * n0) / (SRef - Sm);
-    double f3 = ((SRef - Vn_ref) * density_ref * v1 + (p_star - pressure_ref) * n1) / (SRef - Sm);
+    T rhoE_ref = pressure_ref / (Gamma - 1) + T(0.5) * density_ref * vv;
+    T f0 = (SRef - Vn_ref) * density_ref / (SRef - Sm);
+    T f1 = ((SRef - Vn_ref) * rhoE_ref - pressure_ref * Vn_ref + p_star * Sm) / (SRef - Sm);
+    T f2 = ((SRef - Vn_ref) * density_ref * v0 + (p_star - pressure_ref) * n0) / (SRef - Sm);
+    T f3 = ((SRef - Vn_ref) * density_ref * v1 + (p_star - pressure_ref) * n1) / (SRef - Sm);
     F[0] = f0 * Sm * area;
     F[1] = (f1 + p_star) * Sm * area;
     F[2] = (f2 * Sm + p_star * n0) * area;

@@ -85,42 +85,94 @@ __device__ __forceinline__ bool get_slot(const DevMesh &m, int s, int c, Slot &o
 // ViscousFlow2D.cpp:174-252, Vankatakrishnan_Limiter.hpp:69-200,
 // LaminarFluxStrategy::computeMu (cells), SSTKWStrategy.h:616-722.
 // ---------------------------------------------------------------------------
-template <int FLOW>
+// Mesh arrays by compute type.
+template <typename T>
+__device__ __forceinline__ const T *fnx_of(const DevMesh &m) { return pickc<T>(m.fnx, m.fnxf); }
+template <typename T>
+__device__ __forceinline__ const T *fny_of(const DevMesh &m) { return pickc<T>(m.fny, m.fnyf); }
+template <typename T>
+__device__ __forceinline__ const T *fA_of(const DevMesh &m) { return pickc<T>(m.fA, m.fAf); }
+
+// Offset x_f - x_c of face f from the centre of `cell` (its left cell when
+// `left`): differenced here in FP64, read precomputed in FP32.
+template <typename T>
+__device__ __forceinline__ void face_offset(const DevMesh &m, int cell, int f, bool left, T &v0, T &v1) {
+  if constexpr (std::is_same<T, double>::value) {
+    v0 = m.fcx[f] - m.ccx[cell];
+    v1 = m.fcy[f] - m.ccy[cell];
+  } else {
+    const int nf = m.nf;
+    v0 = left ? m.fdx[f] : m.fdx[2 * nf + f];
+    v1 = left ? m.fdx[nf + f] : m.fdx[3 * nf + f];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_grad: boundary states + gradients + limiter (+ SST F1/F2)
+// Euler2D::updateEdgeValues/solveGradient (Euler2D.cpp:202-240),
+// ViscousFlow2D.cpp:174-252, Vankatakrishnan_Limiter.hpp:69-200,
+// LaminarFluxStrategy::computeMu (cells), SSTKWStrategy.h:616-722.
+// T is the compute type (double; float in the mixed-precision mode, where
+// boundary states stay FP64 and the cell viscosity is FP64).
+// ---------------------------------------------------------------------------
+template <int FLOW, typename T>
 __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int c;
   if (!xcd_item(items, sl, c)) return;
   const int nc = m.nc;
-  const double Gamma = USG_GAMMA;
-  double Uc[4];
-  load4(s.U, nc, c, Uc);
+  const T *fnx = fnx_of<T>(m), *fny = fny_of<T>(m), *fA = fA_of<T>(m);
+  double Ucd[4];
+  load4(s.U, nc, c, Ucd);
+  T Uc[4];
+  for (int j = 0; j < 4; j++) Uc[j] = T(Ucd[j]);
   const double V = m.vol[c];
-  double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  double gT[2] = {0, 0};
-  double tg[4] = {0, 0, 0, 0};
-  double umax[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
-  double umin[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
-  double tUc[2] = {0, 0}, muc = 0;
+  T g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  T gT[2] = {0, 0};
+  T tg[4] = {0, 0, 0, 0};
+  T umax[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
+  T umin[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};
+  T tUc[2] = {0, 0};
+  double muc = 0;
   if (F::sst) {
-    tUc[0] = s.tU[c];
-    tUc[1] = s.tU[nc + c];
+    tUc[0] = T(s.tU[c]);
+    tUc[1] = T(s.tU[nc + c]);
   }
+  // FP32 (mixed) path: face values enter Green-Gauss as differences from the
+  // cell value, formed in FP64 -- sum (phi_f - phi_c) n A equals
+  // sum phi_f n A on a closed cell (sum n A = 0) without cancelling O(1)
+  // terms in single precision.  FP64 path: the reference's sum phi_f n A.
+  constexpr bool kDiff = std::is_same<T, float>::value;
+  double umaxd[4] = {Ucd[0], Ucd[1], Ucd[2], Ucd[3]};
+  double umind[4] = {Ucd[0], Ucd[1], Ucd[2], Ucd[3]};
+  const double Tcd = Ucd[1] / Ucd[0];
   int bad = 0;
   for (int sl = 0; sl < 4; sl++) {
     Slot o;
     if (!get_slot(m, sl, c, o)) break;
-    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
-    double Ue[4], Unb[4];
+    const T n0 = fnx[o.f], n1 = fny[o.f], A = fA[o.f];
+    T Ue[4];
+    double Ued[4];
     if (o.nb >= 0) {
-      load4(s.U, nc, o.nb, Unb);
+      double Unbd[4];
+      load4(s.U, nc, o.nb, Unbd);
       for (int j = 0; j < 4; j++) {
-        Ue[j] = o.left ? (Uc[j] + Unb[j]) / 2 : (Unb[j] + Uc[j]) / 2;
-        umax[j] = smax(umax[j], Unb[j]);
-        umin[j] = smin(umin[j], Unb[j]);
+        if constexpr (kDiff) {
+          Ued[j] = (Unbd[j] + Ucd[j]) / 2;
+          Ue[j] = T((Unbd[j] - Ucd[j]) / 2);
+          umaxd[j] = smax(umaxd[j], Unbd[j]);
+          umind[j] = smin(umind[j], Unbd[j]);
+        } else {
+          const T Unb = T(Unbd[j]);
+          Ue[j] = o.left ? (Uc[j] + Unb) / 2 : (Unb + Uc[j]) / 2;
+          umax[j] = smax(umax[j], Unb);
+          umin[j] = smin(umin[j], Unb);
+        }
       }
     } else {
-      if (!boundary_state<F::visc>(k, o.nb, Uc, n0, n1, Ue)) bad = 1;
+      if (!boundary_state<F::visc>(k, o.nb, Ucd, m.fnx[o.f], m.fny[o.f], Ued)) bad = 1;
+      for (int j = 0; j < 4; j++) Ue[j] = kDiff ? T(Ued[j] - Ucd[j]) : T(Ued[j]);
     }
     // Green-Gauss: + for the left cell, - for the right cell
     if (o.left) {
@@ -135,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
       }
     }
     if (F::visc) {
-      double edgeT = Ue[1] / Ue[0];
+      T edgeT = kDiff ? T(Ued[1] / Ued[0] - Tcd) : Ue[1] / Ue[0];
       if (o.left) {
         gT[0] = gT[0] + n0 * (A * edgeT);
         gT[1] = gT[1] + n1 * (A * edgeT);
@@ -145,21 +197,42 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
       }
       if (F::sst) {
         // ComputeFaceTValue (SSTKWStrategy.h:616-659)
-        double te[2];
-        if (o.nb >= 0) {
-          double t0 = s.tU[o.nb], t1 = s.tU[nc + o.nb];
-          te[0] = o.left ? 0.5 * (tUc[0] + t0) : 0.5 * (t0 + tUc[0]);
-          te[1] = o.left ? 0.5 * (tUc[1] + t1) : 0.5 * (t1 + tUc[1]);
-        } else if (o.nb == kWallD) {
-          double dis = m.wdist[c];
-          te[0] = 0.0;
-          te[1] = 10.0 * 6.0 * sutherland(k, edgeT) / (Ue[0] * SST_BETA1 * dis * dis);
-        } else if (dot2(Ue[2], Ue[3], n0, n1) > 0.0) {
-          te[0] = tUc[0];
-          te[1] = tUc[1];
+        T te[2];
+        if constexpr (kDiff) {
+          const double tc0 = s.tU[c], tc1 = s.tU[nc + c];
+          double ted[2];
+          if (o.nb >= 0) {
+            ted[0] = 0.5 * (s.tU[o.nb] - tc0);
+            ted[1] = 0.5 * (s.tU[nc + o.nb] - tc1);
+          } else if (o.nb == kWallD) {
+            const double dis = m.wdist[c];
+            ted[0] = 0.0 - tc0;
+            ted[1] = 10.0 * 6.0 * sutherland<double>(k, Ued[1] / Ued[0]) / (Ued[0] * SST_BETA1 * dis * dis) - tc1;
+          } else if (dot2(Ued[2], Ued[3], m.fnx[o.f], m.fny[o.f]) > 0.0) {
+            ted[0] = 0.0;
+            ted[1] = 0.0;
+          } else {
+            ted[0] = k.fsnd_turb[0] - tc0;
+            ted[1] = k.fsnd_turb[1] - tc1;
+          }
+          te[0] = T(ted[0]);
+          te[1] = T(ted[1]);
         } else {
-          te[0] = k.fsnd_turb[0];
-          te[1] = k.fsnd_turb[1];
+          if (o.nb >= 0) {
+            T t0 = T(s.tU[o.nb]), t1 = T(s.tU[nc + o.nb]);
+            te[0] = o.left ? T(0.5) * (tUc[0] + t0) : T(0.5) * (t0 + tUc[0]);
+            te[1] = o.left ? T(0.5) * (tUc[1] + t1) : T(0.5) * (t1 + tUc[1]);
+          } else if (o.nb == kWallD) {
+            T dis = T(m.wdist[c]);
+            te[0] = 0.0;
+            te[1] = T(10.0 * 6.0) * sutherland<T>(k, edgeT) / (Ue[0] * T(SST_BETA1) * dis * dis);
+          } else if (dot2(Ue[2], Ue[3], n0, n1) > T(0.0)) {
+            te[0] = tUc[0];
+            te[1] = tUc[1];
+          } else {
+            te[0] = T(k.fsnd_turb[0]);
+            te[1] = T(k.fsnd_turb[1]);
+          }
         }
         for (int q = 0; q < 2; q++) {
           if (o.left) {
@@ -173,65 +246,79 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
       }
     }
   }
-  for (int j = 0; j < 8; j++) g[j] = g[j] / V;
-  for (int j = 0; j < 8; j++) s.grad[j * nc + c] = g[j];
+  const T Vt = T(V);
+  T *G = pick<T>(s.grad, s.gradf);
+  for (int j = 0; j < 8; j++) g[j] = g[j] / Vt;
+  for (int j = 0; j < 8; j++) G[j * nc + c] = g[j];
   if (F::visc) {
-    gT[0] = gT[0] / V;
-    gT[1] = gT[1] / V;
-    s.gradT[c] = gT[0];
-    s.gradT[nc + c] = gT[1];
-    double T = Uc[1] / Uc[0];
-    muc = sutherland(k, T);
+    T *GT = pick<T>(s.gradT, s.gradTf);
+    gT[0] = gT[0] / Vt;
+    gT[1] = gT[1] / Vt;
+    GT[c] = gT[0];
+    GT[nc + c] = gT[1];
+    double Tc = Ucd[1] / Ucd[0];
+    muc = sutherland<double>(k, Tc);
     s.mu[c] = muc;
   }
   if (bad) atomicAdd(&s.err[1], 1);
 
   // Venkatakrishnan limiter: eps^2 = 5^3 * V_left (integer 3/ndim == 1)
-  double d1max[4], d1min[4], ph[4] = {1.0, 1.0, 1.0, 1.0};
+  T d1max[4], d1min[4], ph[4] = {1.0, 1.0, 1.0, 1.0};
   for (int j = 0; j < 4; j++) {
-    d1max[j] = umax[j] - Uc[j];
-    d1min[j] = umin[j] - Uc[j];
+    if constexpr (kDiff) {
+      d1max[j] = T(umaxd[j] - Ucd[j]);
+      d1min[j] = T(umind[j] - Ucd[j]);
+    } else {
+      d1max[j] = umax[j] - Uc[j];
+      d1min[j] = umin[j] - Uc[j];
+    }
   }
-  const double xc = m.ccx[c], yc = m.ccy[c];
   for (int sl = 0; sl < 4; sl++) {
     Slot o;
     if (!get_slot(m, sl, c, o)) break;
-    double v0 = m.fcx[o.f] - xc, v1 = m.fcy[o.f] - yc;
-    double Vl = o.left ? V : m.vol[o.nb];
-    double eps2 = 125.0 * Vl;
+    T v0, v1;
+    face_offset<T>(m, c, o.f, o.left, v0, v1);
+    T Vl = o.left ? Vt : T(m.vol[o.nb]);
+    T eps2 = T(125.0) * Vl;
     for (int j = 0; j < 4; j++) {
-      double delta2 = dot2(g[2 * j], g[2 * j + 1], v0, v1);
+      T delta2 = dot2(g[2 * j], g[2 * j + 1], v0, v1);
       ph[j] = smin(ph[j], venkat(d1max[j], d1min[j], delta2, eps2));
     }
   }
-  for (int j = 0; j < 4; j++) s.phi[j * nc + c] = ph[j];
+  T *P = pick<T>(s.phi, s.phif);
+  for (int j = 0; j < 4; j++) P[j * nc + c] = ph[j];
 
   if (F::sst) {
-    for (int q = 0; q < 4; q++) tg[q] = tg[q] / V;
-    for (int q = 0; q < 4; q++) s.tgrad[q * nc + c] = tg[q];
+    T *TG = pick<T>(s.tgrad, s.tgradf);
+    for (int q = 0; q < 4; q++) tg[q] = tg[q] / Vt;
+    for (int q = 0; q < 4; q++) TG[q * nc + c] = tg[q];
     // ComputeSSTf1f2 (SSTKWStrategy.h:695-722)
-    double gradVal = dot2(tg[0], tg[1], tg[2], tg[3]);
-    double rho = Uc[0], dis = m.wdist[c];
-    double Kk = tUc[0], w = tUc[1];
-    double muL = muc;
-    double CD_kw = smax(2.0 * rho * SST_SW2 / w * gradVal, 1.0e-20);
-    double arg1 = smax(sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
-    arg1 = smin(arg1, 4.0 * rho * SST_SW2 * Kk / (CD_kw * dis * dis));
-    double arg2 = smax(2.0 * sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
-    s.F1[c] = tanh(arg1 * arg1 * arg1 * arg1);
-    s.F2[c] = tanh(arg2 * arg2);
+    T gradVal = dot2(tg[0], tg[1], tg[2], tg[3]);
+    T rho = Uc[0], dis = T(m.wdist[c]);
+    T Kk = tUc[0], w = tUc[1];
+    T muL = T(muc);
+    T CD_kw = smax(T(2.0) * rho * T(SST_SW2) / w * gradVal, T(1.0e-20));
+    T arg1 = smax(sqrt(Kk) / (T(0.09) * w * dis), T(500.0) * muL / (rho * w * dis * dis));
+    arg1 = smin(arg1, T(4.0) * rho * T(SST_SW2) * Kk / (CD_kw * dis * dis));
+    T arg2 = smax(T(2.0) * sqrt(Kk) / (T(0.09) * w * dis), T(500.0) * muL / (rho * w * dis * dis));
+    pick<T>(s.F1, s.F1f)[c] = tanh(arg1 * arg1 * arg1 * arg1);
+    pick<T>(s.F2, s.F2f)[c] = tanh(arg2 * arg2);
   }
 }
 
 // Limited reconstruction of one side: UL = (grad . (x_f - x_c)) * phi + U
 // (Vankatakrishnan_Limiter.hpp:118-125, HLLCFluxStrategy.h:56-65).
-__device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s, int cell, int f,
-                                            const double *Ucell, double *out) {
+template <typename T>
+__device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s, int cell, int f, bool left,
+                                            const T *Ucell, T *out) {
   const int nc = m.nc;
-  double v0 = m.fcx[f] - m.ccx[cell], v1 = m.fcy[f] - m.ccy[cell];
+  const T *G = pickc<T>(s.grad, s.gradf);
+  const T *P = pickc<T>(s.phi, s.phif);
+  T v0, v1;
+  face_offset<T>(m, cell, f, left, v0, v1);
   for (int j = 0; j < 4; j++) {
-    double d2 = dot2(s.grad[(2 * j) * nc + cell], s.grad[(2 * j + 1) * nc + cell], v0, v1);
-    out[j] = d2 * s.phi[j * nc + cell] + Ucell[j];
+    T d2 = dot2(G[(2 * j) * nc + cell], G[(2 * j + 1) * nc + cell], v0, v1);
+    out[j] = d2 * P[j * nc + cell] + Ucell[j];
   }
 }
 
@@ -244,97 +331,114 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
 // Outputs per face: Fc[4], Fv[4] (viscous), Ft[2] (SST), all area-weighted,
 // oriented out of the left cell.
 // ---------------------------------------------------------------------------
-template <int FLOW>
+template <int FLOW, typename T>
 __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int f;
   if (!xcd_item(items, sl, f)) return;
   const int nc = m.nc, nf = m.nf;
-  const double Gamma = USG_GAMMA;
+  const T Gamma = T(USG_GAMMA);
   const int L = m.flc[f], Rr = m.frc[f];
-  const double n0 = m.fnx[f], n1 = m.fny[f], A = m.fA[f];
-  double UL[4], UR[4], Ue[4], Fx[4];
-  load4(s.U, nc, L, UL);
+  const T n0 = fnx_of<T>(m)[f], n1 = fny_of<T>(m)[f], A = fA_of<T>(m)[f];
+  double ULd[4], URd[4] = {0, 0, 0, 0};
+  load4(s.U, nc, L, ULd);
+  T UL[4], UR[4], Ue[4], Fx[4];
+  for (int j = 0; j < 4; j++) UL[j] = T(ULd[j]);
   const bool inner = Rr >= 0;
   if (inner) {
-    load4(s.U, nc, Rr, UR);
-    double Ua[4], Ub[4];
-    reconstruct(m, s, L, f, UL, Ua);
-    reconstruct(m, s, Rr, f, UR, Ub);
+    load4(s.U, nc, Rr, URd);
+    for (int j = 0; j < 4; j++) UR[j] = T(URd[j]);
+    T Ua[4], Ub[4];
+    reconstruct<T>(m, s, L, f, true, UL, Ua);
+    reconstruct<T>(m, s, Rr, f, false, UR, Ub);
     if (!hllc(Ua, Ub, n0, n1, A, Fx)) atomicAdd(&s.err[0], 1);
   } else {
-    boundary_state<F::visc>(k, Rr, UL, n0, n1, Ue);
-    boundary_flux(Rr, Ue, n0, n1, A, Fx);
+    // boundary faces in FP64 in every mode
+    double Ued[4], Fxd[4];
+    boundary_state<F::visc>(k, Rr, ULd, m.fnx[f], m.fny[f], Ued);
+    boundary_flux(Rr, Ued, m.fnx[f], m.fny[f], m.fA[f], Fxd);
+    for (int j = 0; j < 4; j++) {
+      Ue[j] = T(Ued[j]);
+      Fx[j] = T(Fxd[j]);
+    }
   }
   // face record, quantity q at FF[q*nf + f]: flux 0-3, spectral term 4,
   // viscous flux 5-8, viscous spectral term 9, SST 10-11
-  double *const FF = s.FF + f;
+  T *const FF = pick<T>(s.FF, s.FFf) + f;
   for (int j = 0; j < 4; j++) FF[(size_t)j * nf] = Fx[j];
   if (inner)
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
   {
     // spectral radius term (Euler2D::solveSpectralRadius, Euler2D.cpp:242-266)
-    double cs = sqrt(Gamma * Ue[1] / Ue[0]);
-    double Vn = dot2(Ue[2], Ue[3], n0, n1);
+    T cs = sqrt(Gamma * Ue[1] / Ue[0]);
+    T Vn = dot2(Ue[2], Ue[3], n0, n1);
     FF[4 * (size_t)nf] = (fabs(Vn) + cs) * A;
   }
   if (!F::visc) return;
-  double mue = sutherland(k, Ue[1] / Ue[0]);
-  double mutf = 0.0;
+  const T *G = pickc<T>(s.grad, s.gradf);
+  const T *GT = pickc<T>(s.gradT, s.gradTf);
+  T mue = sutherland<T>(k, Ue[1] / Ue[0]);
+  T mutf = 0.0;
   if (F::sst) {
     if (inner) {
-      mutf = 0.5 * (s.mut[L] + s.mut[Rr]);
+      mutf = T(0.5) * (T(s.mut[L]) + T(s.mut[Rr]));
     } else if (Rr == kWallD) {
       mutf = 0.0;
     } else {
-      double Vn = dot2(Ue[2], Ue[3], n0, n1);
-      mutf = Vn > 0.0 ? s.mut[L] : k.fsnd_mut;
+      T Vn = dot2(Ue[2], Ue[3], n0, n1);
+      mutf = Vn > T(0.0) ? T(s.mut[L]) : T(k.fsnd_mut);
     }
   }
-  double mu_at_face = F::sst ? mue + mutf : mue;
-  double mpr = mue / k.pr;
-  if (F::sst) mpr = mpr + mutf / k.prt;
-  double efficient = 2.0 / 3.0 * mu_at_face;
-  double gE[3][2];
-  double l2r0 = 0, l2r1 = 0, l2sq = 0;
+  T mu_at_face = F::sst ? mue + mutf : mue;
+  T mpr = mue / T(k.pr);
+  if (F::sst) mpr = mpr + mutf / T(k.prt);
+  T efficient = T(2.0) / T(3.0) * mu_at_face;
+  T gE[3][2];
+  T l2r0 = 0, l2r1 = 0, l2sq = 0;
   if (inner) {
-    l2r0 = m.ccx[Rr] - m.ccx[L];
-    l2r1 = m.ccy[Rr] - m.ccy[L];
+    if constexpr (std::is_same<T, double>::value) {
+      l2r0 = m.ccx[Rr] - m.ccx[L];
+      l2r1 = m.ccy[Rr] - m.ccy[L];
+    } else {
+      l2r0 = m.fdx[4 * nf + f];
+      l2r1 = m.fdx[5 * nf + f];
+    }
     l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
     for (int j = 0; j < 2; j++) {
-      double a0 = (s.grad[(4 + 2 * j) * nc + L] + s.grad[(4 + 2 * j) * nc + Rr]) * 0.5;
-      double a1 = (s.grad[(5 + 2 * j) * nc + L] + s.grad[(5 + 2 * j) * nc + Rr]) * 0.5;
-      double sc = (UR[2 + j] - UL[2 + j] - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+      T a0 = (G[(4 + 2 * j) * nc + L] + G[(4 + 2 * j) * nc + Rr]) * T(0.5);
+      T a1 = (G[(5 + 2 * j) * nc + L] + G[(5 + 2 * j) * nc + Rr]) * T(0.5);
+      // right - left differences are formed in FP64 (exact for T = double)
+      T sc = (T(URd[2 + j] - ULd[2 + j]) - dot2(a0, a1, l2r0, l2r1)) / l2sq;
       gE[j][0] = a0 + l2r0 * sc;
       gE[j][1] = a1 + l2r1 * sc;
     }
-    double a0 = (s.gradT[L] + s.gradT[Rr]) * 0.5;
-    double a1 = (s.gradT[nc + L] + s.gradT[nc + Rr]) * 0.5;
-    double rightT = UR[1] / UR[0];
-    double leftT = UL[1] / UL[0];
-    double sc = (rightT - leftT - dot2(a0, a1, l2r0, l2r1)) / l2sq;
+    T a0 = (GT[L] + GT[Rr]) * T(0.5);
+    T a1 = (GT[nc + L] + GT[nc + Rr]) * T(0.5);
+    double rightT = URd[1] / URd[0];
+    double leftT = ULd[1] / ULd[0];
+    T sc = (T(rightT - leftT) - dot2(a0, a1, l2r0, l2r1)) / l2sq;
     gE[2][0] = a0 + l2r0 * sc;
     gE[2][1] = a1 + l2r1 * sc;
   } else {
-    gE[0][0] = s.grad[4 * nc + L];
-    gE[0][1] = s.grad[5 * nc + L];
-    gE[1][0] = s.grad[6 * nc + L];
-    gE[1][1] = s.grad[7 * nc + L];
+    gE[0][0] = G[4 * nc + L];
+    gE[0][1] = G[5 * nc + L];
+    gE[1][0] = G[6 * nc + L];
+    gE[1][1] = G[7 * nc + L];
     if (Rr == kWallD) {
       gE[2][0] = 0;
       gE[2][1] = 0;
     } else {
-      gE[2][0] = s.gradT[L];
-      gE[2][1] = s.gradT[nc + L];
+      gE[2][0] = GT[L];
+      gE[2][1] = GT[nc + L];
     }
   }
-  double taoxx = efficient * (2.0 * gE[0][0] - gE[1][1]);
-  double taoyy = efficient * (2.0 * gE[1][1] - gE[0][0]);
-  double taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
-  double qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
-  double qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
-  double Fv[4];
+  T taoxx = efficient * (T(2.0) * gE[0][0] - gE[1][1]);
+  T taoyy = efficient * (T(2.0) * gE[1][1] - gE[0][0]);
+  T taoxy = mu_at_face * (gE[0][1] + gE[1][0]);
+  T qx = -Gamma / (Gamma - 1) * (mpr)*gE[2][0];
+  T qy = -Gamma / (Gamma - 1) * (mpr)*gE[2][1];
+  T Fv[4];
   Fv[0] = 0.0;
   Fv[1] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
   Fv[2] = (taoxx * n0 + taoxy * n1) * A;
@@ -342,64 +446,67 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   for (int j = 0; j < 4; j++) FF[(size_t)(5 + j) * nf] = Fv[j];
   // viscous spectral radius term (ViscousFlow2D.cpp:243-249); each cell
   // divides it by its own volume
-  FF[9 * (size_t)nf] = smax(4 / (3.0 * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
+  FF[9 * (size_t)nf] = smax(T(4) / (T(3.0) * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
   if (!F::sst) return;
   // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
-  const double Vn = dot2(Ue[2], Ue[3], n0, n1);
-  double tl[2] = {s.tU[L], s.tU[nc + L]}, tr[2] = {0, 0}, te[2];
+  const T Vn = dot2(Ue[2], Ue[3], n0, n1);
+  T tl[2] = {T(s.tU[L]), T(s.tU[nc + L])}, tr[2] = {0, 0}, te[2];
   if (inner) {
-    tr[0] = s.tU[Rr];
-    tr[1] = s.tU[nc + Rr];
-    te[0] = 0.5 * (tl[0] + tr[0]);
-    te[1] = 0.5 * (tl[1] + tr[1]);
+    tr[0] = T(s.tU[Rr]);
+    tr[1] = T(s.tU[nc + Rr]);
+    te[0] = T(0.5) * (tl[0] + tr[0]);
+    te[1] = T(0.5) * (tl[1] + tr[1]);
   } else if (Rr == kWallD) {
-    double dis = m.wdist[L];
+    T dis = T(m.wdist[L]);
     te[0] = 0.0;
-    te[1] = 10.0 * 6.0 * mue / (Ue[0] * SST_BETA1 * dis * dis);
-  } else if (Vn > 0.0) {
+    te[1] = T(10.0 * 6.0) * mue / (Ue[0] * T(SST_BETA1) * dis * dis);
+  } else if (Vn > T(0.0)) {
     te[0] = tl[0];
     te[1] = tl[1];
   } else {
-    te[0] = k.fsnd_turb[0];
-    te[1] = k.fsnd_turb[1];
+    te[0] = T(k.fsnd_turb[0]);
+    te[1] = T(k.fsnd_turb[1]);
   }
   // convective (SolveConvectiveTFlux :783-831)
-  double Fct[2];
+  T Fct[2];
   if (inner) {
-    double Vp = 0.5 * (Vn + fabs(Vn));
-    double Vs = 0.5 * (Vn - fabs(Vn));
+    T Vp = T(0.5) * (Vn + fabs(Vn));
+    T Vs = T(0.5) * (Vn - fabs(Vn));
     for (int q = 0; q < 2; q++) Fct[q] = tl[q] * UL[0] * Vp * A + tr[q] * UR[0] * Vs * A;
   } else {
     for (int q = 0; q < 2; q++) Fct[q] = Vn * Ue[0] * te[q] * A;
   }
   // diffusive (SolveViscousTFlux :834-920)
-  double face_f1;
-  double gq[2][2];
+  const T *F1 = pickc<T>(s.F1, s.F1f);
+  const T *TG = pickc<T>(s.tgrad, s.tgradf);
+  T face_f1;
+  T gq[2][2];
   if (inner) {
-    face_f1 = 0.5 * (s.F1[L] + s.F1[Rr]);
+    face_f1 = T(0.5) * (F1[L] + F1[Rr]);
     for (int q = 0; q < 2; q++) {
-      gq[q][0] = (s.tgrad[(2 * q) * nc + L] + s.tgrad[(2 * q) * nc + Rr]) * 0.5;
-      gq[q][1] = (s.tgrad[(2 * q + 1) * nc + L] + s.tgrad[(2 * q + 1) * nc + Rr]) * 0.5;
+      gq[q][0] = (TG[(2 * q) * nc + L] + TG[(2 * q) * nc + Rr]) * T(0.5);
+      gq[q][1] = (TG[(2 * q + 1) * nc + L] + TG[(2 * q + 1) * nc + Rr]) * T(0.5);
     }
     for (int q = 0; q < 2; q++) {
-      gq[q][0] = gq[q][0] + ((tr[q] - tl[q]) - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r0 / l2sq;
-      gq[q][1] = gq[q][1] + ((tr[q] - tl[q]) - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r1 / l2sq;
+      const T dq = T(s.tU[q * nc + Rr] - s.tU[q * nc + L]);
+      gq[q][0] = gq[q][0] + (dq - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r0 / l2sq;
+      gq[q][1] = gq[q][1] + (dq - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r1 / l2sq;
     }
   } else {
-    face_f1 = s.F1[L];
+    face_f1 = F1[L];
     for (int q = 0; q < 2; q++) {
-      gq[q][0] = s.tgrad[(2 * q) * nc + L];
-      gq[q][1] = s.tgrad[(2 * q + 1) * nc + L];
+      gq[q][0] = TG[(2 * q) * nc + L];
+      gq[q][1] = TG[(2 * q + 1) * nc + L];
     }
   }
-  double sigma_k = face_f1 * SST_SK1 + (1 - face_f1) * SST_SK2;
-  double sigma_w = face_f1 * SST_SW1 + (1 - face_f1) * SST_SW2;
-  double txx_k = (mue + sigma_k * mutf) * gq[0][0];
-  double tyy_k = (mue + sigma_k * mutf) * gq[0][1];
-  double txx_w = (mue + sigma_w * mutf) * gq[1][0];
-  double tyy_w = (mue + sigma_w * mutf) * gq[1][1];
-  double Fv0 = txx_k * n0 + tyy_k * n1;
-  double Fv1 = txx_w * n0 + tyy_w * n1;
+  T sigma_k = face_f1 * T(SST_SK1) + (1 - face_f1) * T(SST_SK2);
+  T sigma_w = face_f1 * T(SST_SW1) + (1 - face_f1) * T(SST_SW2);
+  T txx_k = (mue + sigma_k * mutf) * gq[0][0];
+  T tyy_k = (mue + sigma_k * mutf) * gq[0][1];
+  T txx_w = (mue + sigma_w * mutf) * gq[1][0];
+  T tyy_w = (mue + sigma_w * mutf) * gq[1][1];
+  T Fv0 = txx_k * n0 + tyy_k * n1;
+  T Fv1 = txx_w * n0 + tyy_w * n1;
   Fv0 *= A;
   Fv1 *= A;
   FF[10 * (size_t)nf] = Fct[0] - Fv0;
@@ -416,7 +523,7 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
 // ViscousFlow2D.h:223-242), D (SolveDiag, TopologyHolderStrategy.cpp:190-215),
 // the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
 // ---------------------------------------------------------------------------
-template <int FLOW>
+template <int FLOW, typename T>
 __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
@@ -431,25 +538,26 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
     fs[ns++] = e;
   }
   // faces in ascending order, each added with this cell's sign (left +,
-  // right -): the reference's scatter order
+  // right -): the reference's scatter order.  Sums are FP64 in every mode.
+  const T *const FFb = pickc<T>(s.FF, s.FFf);
   double Rc[4] = {0, 0, 0, 0};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    const double *FF = s.FF + (e >= 0 ? e : -e - 1);
+    const T *FF = FFb + (e >= 0 ? e : -e - 1);
     if (e >= 0) {
-      for (int j = 0; j < 4; j++) Rc[j] += FF[(size_t)j * nf];
+      for (int j = 0; j < 4; j++) Rc[j] += double(FF[(size_t)j * nf]);
     } else {
-      for (int j = 0; j < 4; j++) Rc[j] -= FF[(size_t)j * nf];
+      for (int j = 0; j < 4; j++) Rc[j] -= double(FF[(size_t)j * nf]);
     }
   }
   if (F::visc) {
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
-      const double *FF = s.FF + (e >= 0 ? e : -e - 1);
+      const T *FF = FFb + (e >= 0 ? e : -e - 1);
       if (e >= 0) {
-        for (int j = 0; j < 4; j++) Rc[j] -= FF[(size_t)(5 + j) * nf];
+        for (int j = 0; j < 4; j++) Rc[j] -= double(FF[(size_t)(5 + j) * nf]);
       } else {
-        for (int j = 0; j < 4; j++) Rc[j] += FF[(size_t)(5 + j) * nf];
+        for (int j = 0; j < 4; j++) Rc[j] += double(FF[(size_t)(5 + j) * nf]);
       }
     }
   }
@@ -457,13 +565,13 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   double lamc = 0, lamv = 0;
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    lamc += s.FF[(e >= 0 ? e : -e - 1) + 4 * (size_t)nf];
+    lamc += double(FFb[(e >= 0 ? e : -e - 1) + 4 * (size_t)nf]);
   }
   s.lamc[c] = lamc;
   if (F::visc) {
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
-      lamv += s.FF[(e >= 0 ? e : -e - 1) + 9 * (size_t)nf] / V;
+      lamv += double(FFb[(e >= 0 ? e : -e - 1) + 9 * (size_t)nf]) / V;
     }
     s.lamv[c] = lamv;
   }
@@ -485,12 +593,13 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   }
   if (!F::sst) return;
   double tUc[2] = {s.tU[c], s.tU[nc + c]};
+  const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
   double tgc[4];
-  for (int q = 0; q < 4; q++) tgc[q] = s.tgrad[q * nc + c];
-  const double F1c = s.F1[c], mutc = s.mut[c];
+  for (int q = 0; q < 4; q++) tgc[q] = double(TG[q * nc + c]);
+  const double F1c = double(pickc<T>(s.F1, s.F1f)[c]), mutc = s.mut[c];
   const double rho = s.U[c];
-  double gux = s.grad[4 * nc + c], guy = s.grad[5 * nc + c];
-  double gvx = s.grad[6 * nc + c], gvy = s.grad[7 * nc + c];
+  double gux = double(G[4 * nc + c]), guy = double(G[5 * nc + c]);
+  double gvx = double(G[6 * nc + c]), gvy = double(G[7 * nc + c]);
   double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
   double ddls1 = (guy + gvx) * (guy + gvx);
   double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
@@ -504,13 +613,13 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   double trc[2] = {-(Q0 * V), -(Q1 * V)};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    const double *FF = s.FF + (e >= 0 ? e : -e - 1);
+    const T *FF = FFb + (e >= 0 ? e : -e - 1);
     if (e >= 0) {
-      trc[0] += FF[10 * (size_t)nf];
-      trc[1] += FF[11 * (size_t)nf];
+      trc[0] += double(FF[10 * (size_t)nf]);
+      trc[1] += double(FF[11 * (size_t)nf]);
     } else {
-      trc[0] -= FF[10 * (size_t)nf];
-      trc[1] -= FF[11 * (size_t)nf];
+      trc[0] -= double(FF[10 * (size_t)nf]);
+      trc[1] -= double(FF[11 * (size_t)nf]);
     }
   }
   s.tR[c] = trc[0];
@@ -532,7 +641,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
 // UpdatePrimitiveVariable (LUSGS_Integrator.cpp:242-255) into the second
 // state buffer, and the SST update (SSTKWStrategy.h:194-223).
 // ---------------------------------------------------------------------------
-template <int FLOW, bool FWD>
+template <int FLOW, bool FWD, typename T>
 __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
                                                  const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
@@ -631,8 +740,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
       s.tWn[nc + c] = tw[1];
       s.tUn[c] = tu[0];
       s.tUn[nc + c] = tu[1];
-      double curl = fabs(s.grad[6 * nc + c] - s.grad[5 * nc + c]);
-      double h = smax(SST_A1 * tw[1] / rho, s.F2[c] * curl);
+      const T *G = pickc<T>(s.grad, s.gradf);
+      double curl = fabs(double(G[6 * nc + c]) - double(G[5 * nc + c]));
+      double h = smax(SST_A1 * tw[1] / rho, double(pickc<T>(s.F2, s.F2f)[c]) * curl);
       s.mutn[c] = SST_A1 * tw[0] / h;
     }
   }
@@ -923,25 +1033,46 @@ static int xgrid(const XSlice &sl) {
   return kXcd * std::max(1, (mx + kBlock - 1) / kBlock);
 }
 
-template <int FLOW>
+template <int FLOW, typename T>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
-  hipLaunchKernelGGL(k_grad<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
-                     cs);
+  hipLaunchKernelGGL((k_grad<FLOW, T>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                     c->d_cell_list, cs);
   if (ea) hipEventRecord(ea, c->stream);
-  hipLaunchKernelGGL(k_face<FLOW>, dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_face_list,
-                     fs);
+  hipLaunchKernelGGL((k_face<FLOW, T>), dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                     c->d_face_list, fs);
   if (eb) hipEventRecord(eb, c->stream);
-  hipLaunchKernelGGL(k_cell<FLOW>, dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, c->d_cell_list,
-                     cs);
+  hipLaunchKernelGGL((k_cell<FLOW, T>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+                     c->d_cell_list, cs);
+}
+
+template <typename T>
+static void launch_residual_t(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
+  switch (c->prm.flow) {
+    case USG_EULER: launch_residual<USG_EULER, T>(c, ea, eb); break;
+    case USG_LAMINAR: launch_residual<USG_LAMINAR, T>(c, ea, eb); break;
+    default: launch_residual<USG_SST, T>(c, ea, eb); break;
+  }
 }
 
 static void launch_residual_events(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
-  switch (c->prm.flow) {
-    case USG_EULER: launch_residual<USG_EULER>(c, ea, eb); break;
-    case USG_LAMINAR: launch_residual<USG_LAMINAR>(c, ea, eb); break;
-    default: launch_residual<USG_SST>(c, ea, eb); break;
+  if (c->prm.precision == USG_MIXED)
+    launch_residual_t<float>(c, ea, eb);
+  else
+    launch_residual_t<double>(c, ea, eb);
+}
+
+// A pooled event pair for one timed launch (nullptr pair when the pool cannot grow).
+static std::pair<hipEvent_t, hipEvent_t> next_events(usg_ctx *c) {
+  if (c->ev_used == c->ev.size()) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      if (a) hipEventDestroy(a);
+      return {nullptr, nullptr};
+    }
+    c->ev.push_back({a, b});
   }
+  return c->ev[c->ev_used++];
 }
 
 // Eager residual; in timing mode the flux kernel is bracketed by a pooled event pair.
@@ -950,30 +1081,33 @@ static void launch_residual_any(usg_ctx *c) {
     launch_residual_events(c, nullptr, nullptr);
     return;
   }
-  if (c->ev_used == c->ev.size()) {
-    hipEvent_t a, b;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
-    c->ev.push_back({a, b});
-  }
-  launch_residual_events(c, c->ev[c->ev_used].first, c->ev[c->ev_used].second);
-  c->ev_used++;
+  auto e = next_events(c);
+  launch_residual_events(c, e.first, e.second);
 }
 
-template <int FLOW>
+template <int FLOW, typename T>
 static void launch_sweeps(usg_ctx *c) {
   const HostMesh &h = c->hm;
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nw; w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, true>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+    hipLaunchKernelGGL((k_sweep<FLOW, true, T>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                        c->d_fwd, sl);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nb; w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, false>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+    hipLaunchKernelGGL((k_sweep<FLOW, false, T>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                        c->d_bwd, sl);
+  }
+}
+
+template <typename T>
+static void launch_sweeps_t(usg_ctx *c) {
+  switch (c->prm.flow) {
+    case USG_EULER: launch_sweeps<USG_EULER, T>(c); break;
+    case USG_LAMINAR: launch_sweeps<USG_LAMINAR, T>(c); break;
+    default: launch_sweeps<USG_SST, T>(c); break;
   }
 }
 
@@ -1044,7 +1178,6 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   *out = nullptr;
   if (params->flow < USG_EULER || params->flow > USG_SST) return fail(USG_ERR_ARG, "Undefined Flow Type");
   if (params->precision != USG_FP64 && params->precision != USG_MIXED) return fail(USG_ERR_ARG, "bad precision");
-  if (params->precision == USG_MIXED) return fail(USG_ERR_ARG, "USG_MIXED is not built into this library");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
   usg_ctx *c = new usg_ctx();
@@ -1117,24 +1250,56 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
   CK(upload(c, &c->d_wall_cells, wc.data(), wc.size()));
   DevState &s = c->ds;
   const bool visc = params->flow != USG_EULER, sst = params->flow == USG_SST;
+  const bool mixed = params->precision == USG_MIXED;
+  if (mixed) {
+    // FP32 face geometry; the offsets are differenced in FP64 first
+    std::vector<float> nxf(nf), nyf(nf), af(nf), fd(6 * (size_t)nf, 0.0f);
+    for (int f = 0; f < nf; f++) {
+      nxf[f] = (float)fnx[f];
+      nyf[f] = (float)fny[f];
+      af[f] = (float)h.fa[f];
+      const int L = h.flc[f], R = h.frc[f];
+      fd[f] = (float)(fcx[f] - ccx[L]);
+      fd[nf + f] = (float)(fcy[f] - ccy[L]);
+      if (R >= 0) {
+        fd[2 * (size_t)nf + f] = (float)(fcx[f] - ccx[R]);
+        fd[3 * (size_t)nf + f] = (float)(fcy[f] - ccy[R]);
+        fd[4 * (size_t)nf + f] = (float)(ccx[R] - ccx[L]);
+        fd[5 * (size_t)nf + f] = (float)(ccy[R] - ccy[L]);
+      }
+    }
+    CK(upload(c, (float **)&dm.fnxf, nxf.data(), nf));
+    CK(upload(c, (float **)&dm.fnyf, nyf.data(), nf));
+    CK(upload(c, (float **)&dm.fAf, af.data(), nf));
+    CK(upload(c, (float **)&dm.fdx, fd.data(), fd.size()));
+  }
   CK(dalloc(c, &s.U, 4 * (size_t)nc));
   CK(dalloc(c, &s.W, 4 * (size_t)nc));
   CK(dalloc(c, &s.Un, 4 * (size_t)nc));
   CK(dalloc(c, &s.Wn, 4 * (size_t)nc));
-  CK(dalloc(c, &s.grad, 8 * (size_t)nc));
+  if (mixed) {
+    CK(dalloc(c, &s.gradf, 8 * (size_t)nc));
+    CK(dalloc(c, &s.phif, 4 * (size_t)nc));
+    CK(dalloc(c, &s.FFf, (size_t)slot_q(params->flow) * nf));
+  } else {
+    CK(dalloc(c, &s.grad, 8 * (size_t)nc));
+    CK(dalloc(c, &s.phi, 4 * (size_t)nc));
+    CK(dalloc(c, &s.FF, (size_t)slot_q(params->flow) * nf));
+  }
   CK(dalloc(c, &s.lamc, nc));
-  CK(dalloc(c, &s.phi, 4 * (size_t)nc));
   CK(dalloc(c, &s.R, 4 * (size_t)nc));
   CK(dalloc(c, &s.dW1, 4 * (size_t)nc));
   CK(dalloc(c, &s.dW, 4 * (size_t)nc));
   CK(dalloc(c, &s.dt, nc));
   CK(dalloc(c, &s.dt_rk, nc));
   CK(dalloc(c, &s.W0, 4 * (size_t)nc));
-  CK(dalloc(c, &s.FF, (size_t)slot_q(params->flow) * nf));
   CK(dalloc(c, &s.D, nc));
   CK(dalloc(c, &s.err, 2));
   if (visc) {
-    CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
+    if (mixed)
+      CK(dalloc(c, &s.gradTf, 2 * (size_t)nc));
+    else
+      CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
     CK(dalloc(c, &s.lamv, nc));
     CK(dalloc(c, &s.mu, nc));
     CK(dalloc(c, &s.mprc, nc));
@@ -1144,9 +1309,15 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
     CK(dalloc(c, &s.tW, 2 * (size_t)nc));
     CK(dalloc(c, &s.tUn, 2 * (size_t)nc));
     CK(dalloc(c, &s.tWn, 2 * (size_t)nc));
-    CK(dalloc(c, &s.tgrad, 4 * (size_t)nc));
-    CK(dalloc(c, &s.F1, nc));
-    CK(dalloc(c, &s.F2, nc));
+    if (mixed) {
+      CK(dalloc(c, &s.tgradf, 4 * (size_t)nc));
+      CK(dalloc(c, &s.F1f, nc));
+      CK(dalloc(c, &s.F2f, nc));
+    } else {
+      CK(dalloc(c, &s.tgrad, 4 * (size_t)nc));
+      CK(dalloc(c, &s.F1, nc));
+      CK(dalloc(c, &s.F2, nc));
+    }
     CK(dalloc(c, &s.mut, nc));
     CK(dalloc(c, &s.mutn, nc));
     CK(dalloc(c, &s.tR, 2 * (size_t)nc));
@@ -1222,11 +1393,10 @@ int usg_residual(usg_ctx *c) {
 }
 
 static void launch_sweeps_any(usg_ctx *c) {
-  switch (c->prm.flow) {
-    case USG_EULER: launch_sweeps<USG_EULER>(c); break;
-    case USG_LAMINAR: launch_sweeps<USG_LAMINAR>(c); break;
-    default: launch_sweeps<USG_SST>(c); break;
-  }
+  if (c->prm.precision == USG_MIXED)
+    launch_sweeps_t<float>(c);
+  else
+    launch_sweeps_t<double>(c);
 }
 
 static void launch_norm_ring(usg_ctx *c) {
@@ -1311,19 +1481,9 @@ int usg_lusgs_step(usg_ctx *c, int n, double *res_hist) {
         HIPCHK(hipGraphLaunch(c->gexec[slot], c->stream));
         i += 2;
       } else {
-        hipEvent_t ea = nullptr, eb = nullptr;
-        if (c->timing) {
-          if (c->ev_used == c->ev.size()) {
-            hipEvent_t a = nullptr, b = nullptr;
-            HIPCHK(hipEventCreate(&a));
-            HIPCHK(hipEventCreate(&b));
-            c->ev.push_back({a, b});
-          }
-          ea = c->ev[c->ev_used].first;
-          eb = c->ev[c->ev_used].second;
-          c->ev_used++;
-        }
-        one_iteration(c, ea, eb);
+        std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+        if (c->timing) e = next_events(c);
+        one_iteration(c, e.first, e.second);
         i += 1;
       }
     }
@@ -1361,6 +1521,18 @@ int usg_rk5_step(usg_ctx *c, int n, double *res_hist) {
   return check_err(c);
 }
 
+static int soa_to_aos(usg_ctx *c, const float *dev, int nv, double *out, long count, bool has) {
+  const int nc = c->hm.nc;
+  if (count != (long)nv * nc) return fail(USG_ERR_ARG, "count mismatch");
+  if (!has || !dev) return fail(USG_ERR_ARG, "field not available for this flow type");
+  std::vector<float> tmp((size_t)nv * nc);
+  HIPCHK(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < nv; j++) out[(size_t)i * nv + j] = (double)tmp[(size_t)j * nc + i];
+  return USG_OK;
+}
+
 static int soa_to_aos(usg_ctx *c, const double *dev, int nv, double *out, long count, bool has) {
   const int nc = c->hm.nc;
   if (count != (long)nv * nc) return fail(USG_ERR_ARG, "count mismatch");
@@ -1381,17 +1553,20 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
     case USG_F_PRIM: return soa_to_aos(c, s.U, 4, out, count, true);
     case USG_F_CONS: return soa_to_aos(c, s.W, 4, out, count, true);
     case USG_F_RES: return soa_to_aos(c, s.R, 4, out, count, true);
-    case USG_F_GRAD: return soa_to_aos(c, s.grad, 8, out, count, true);
+    case USG_F_GRAD:
+      return s.gradf ? soa_to_aos(c, s.gradf, 8, out, count, true) : soa_to_aos(c, s.grad, 8, out, count, true);
     case USG_F_LAMC: return soa_to_aos(c, s.lamc, 1, out, count, true);
-    case USG_F_PHI: return soa_to_aos(c, s.phi, 4, out, count, true);
+    case USG_F_PHI:
+      return s.phif ? soa_to_aos(c, s.phif, 4, out, count, true) : soa_to_aos(c, s.phi, 4, out, count, true);
     case USG_F_DW1: return soa_to_aos(c, s.dW1, 4, out, count, true);
     case USG_F_DW: return soa_to_aos(c, s.dW, 4, out, count, true);
     case USG_F_LAMV: return soa_to_aos(c, s.lamv, 1, out, count, visc);
-    case USG_F_GRADT: return soa_to_aos(c, s.gradT, 2, out, count, visc);
+    case USG_F_GRADT:
+      return s.gradTf ? soa_to_aos(c, s.gradTf, 2, out, count, visc) : soa_to_aos(c, s.gradT, 2, out, count, visc);
     case USG_F_TURB: return soa_to_aos(c, s.tU, 2, out, count, sst);
     case USG_F_TRES: return soa_to_aos(c, s.tR, 2, out, count, sst);
     case USG_F_MUT: return soa_to_aos(c, s.mut, 1, out, count, sst);
-    case USG_F_F1: return soa_to_aos(c, s.F1, 1, out, count, sst);
+    case USG_F_F1: return s.F1f ? soa_to_aos(c, s.F1f, 1, out, count, sst) : soa_to_aos(c, s.F1, 1, out, count, sst);
     case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
     case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
     default: return fail(USG_ERR_ARG, "unknown field");
