@@ -84,6 +84,21 @@ typedef struct usg_ctx usg_ctx;
  * free stream (TopologyHolderStrategy.cpp:325-351) and the LU-SGS wave
  * schedule, uploads SoA arrays.  Contexts are independent. */
 int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out);
+/* Batched many-case context: ncases independent meshes with their own free
+ * streams (mach, aoa, reynolds may differ; flow type, precision and CFL
+ * numbers must agree) solved together as one block-diagonal union, cells
+ * and faces case-major.  Each case keeps its own geometry, wall distance
+ * and numbering, so its results equal a usg_ctx_create context's bit for
+ * bit; the union's LU-SGS waves are the cases' waves side by side, so a
+ * batch needs no more launches per iteration than one case.  Replaces
+ * running WrapSolver (SolverWrapper.h:7-95) once per case, as Orchestra
+ * does per block (Orchestra.cpp:121-153).  Everything below then works on
+ * the union: fields are case-major, and the per-case outputs
+ * (res_hist, usg_res_l2, usg_forces) hold one value per case. */
+int usg_batch_create(const usg_mesh_desc *meshes, const usg_params *params, int ncases, usg_ctx **out);
+/* Number of cases and, if cell_off is not NULL, their cell ranges
+ * (cell_off[ncases+1]: case i owns cells [cell_off[i], cell_off[i+1])). */
+int usg_case_count(usg_ctx *ctx, int *ncases, int *cell_off);
 int usg_ctx_destroy(usg_ctx *ctx);
 /* hipStream_t to launch on (NULL = the context's own stream). */
 int usg_set_stream(usg_ctx *ctx, void *hip_stream);
@@ -102,16 +117,19 @@ int usg_set_prim(usg_ctx *ctx, const double *U, const double *turb);
 int usg_residual(usg_ctx *ctx);
 /* n LU-SGS pseudo-time iterations: LUSGSIntegrator::singleStepSerial /
  * UpdateSerial (LUSGS_Integrator.cpp:124-129,228-255).  res_hist (may be
- * NULL) receives ||R_rho/V||_2 of each iteration's residual. */
+ * NULL) receives ||R_rho/V||_2 of each iteration's residual, per case:
+ * res_hist[it * ncases + case]. */
 int usg_lusgs_step(usg_ctx *ctx, int n, double *res_hist);
-/* n RK5 steps: RungeKuttaStrategy::singleStepSerial (RungeKuttaStrategy.h:28-72). */
+/* n RK5 steps: RungeKuttaStrategy::singleStepSerial (RungeKuttaStrategy.h:28-72);
+ * res_hist as for usg_lusgs_step (the stage-0 residual of each step). */
 int usg_rk5_step(usg_ctx *ctx, int n, double *res_hist);
 
 /* Copy a field to the host (AoS layout of the reference); count = elements. */
 int usg_get_field(usg_ctx *ctx, int field, double *out, long count);
-/* ||R_rho / V||_2 of the last residual (the build's convergence norm). */
+/* ||R_rho / V||_2 of the last residual (the convergence norm), out[ncases]. */
 int usg_res_l2(usg_ctx *ctx, double *out);
-/* Lift and drag coefficients from the wall faces of the last residual. */
+/* Lift and drag coefficients of the current state from the wall faces'
+ * pressure force, cl[ncases], cd[ncases]. */
 int usg_forces(usg_ctx *ctx, double *cl, double *cd);
 
 /* Mesh facts: number of LU-SGS waves (forward, backward) of the numbering. */

@@ -64,3 +64,14 @@ def test_mesh_build_rejects_bad_tags():
     bad = usg._mesh.Mesh(m.xy, m.fnode, m.flc, frc, m.nc, "bad")
     with pytest.raises(usg.UsgError):
         usg.mesh_geometry(bad)
+
+
+def test_batch_rejects_mixed_flow_types():
+    m = part2()
+    d, keep = usg._desc(m)
+    descs = (usg._MeshDesc * 2)(d, d)
+    params = (usg._Params * 2)(_params(usg.EULER), _params(usg.LAMINAR))
+    h = C.c_void_p()
+    assert usg.lib().usg_batch_create(descs, params, 2, C.byref(h)) != 0 and not h.value
+    assert b"share flow type" in usg.lib().usg_last_error()
+    assert usg.lib().usg_batch_create(descs, params, 0, C.byref(h)) != 0

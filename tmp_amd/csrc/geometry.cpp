@@ -332,7 +332,96 @@ std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int 
   }
   waves(m);
   xcd_lists(m);
+  m.case_cell = {0, nc};
+  m.case_face = {0, nf};
+  m.case_wall = {0, (int)m.wall_faces.size()};
   return std::string();
+}
+
+void merge_meshes(const std::vector<HostMesh> &cases, HostMesh &u) {
+  u = HostMesh();
+  const int nk = (int)cases.size();
+  u.case_cell.assign(nk + 1, 0);
+  u.case_face.assign(nk + 1, 0);
+  u.case_wall.assign(nk + 1, 0);
+  std::vector<int> noff(nk + 1, 0);
+  for (int i = 0; i < nk; i++) {
+    noff[i + 1] = noff[i] + cases[i].nn;
+    u.case_cell[i + 1] = u.case_cell[i] + cases[i].nc;
+    u.case_face[i + 1] = u.case_face[i] + cases[i].nf;
+    u.case_wall[i + 1] = u.case_wall[i] + (int)cases[i].wall_faces.size();
+  }
+  u.nn = noff[nk];
+  u.nc = u.case_cell[nk];
+  u.nf = u.case_face[nk];
+  auto shift = [](std::vector<int> &dst, const std::vector<int> &src, int off, bool keep_negative) {
+    for (int v : src) dst.push_back(keep_negative && v < 0 ? v : v + off);
+  };
+  for (int i = 0; i < nk; i++) {
+    const HostMesh &c = cases[i];
+    const int co = u.case_cell[i], fo = u.case_face[i];
+    u.xy.insert(u.xy.end(), c.xy.begin(), c.xy.end());
+    shift(u.fnode, c.fnode, noff[i], false);
+    shift(u.flc, c.flc, co, false);
+    shift(u.frc, c.frc, co, true);
+    for (int v : c.cnode) u.cnode.push_back(v < 0 ? v : v + noff[i]);
+    u.cnv.insert(u.cnv.end(), c.cnv.begin(), c.cnv.end());
+    for (int v : c.cface) u.cface.push_back(v == kNoFace ? v : v + fo);
+    u.cnf.insert(u.cnf.end(), c.cnf.begin(), c.cnf.end());
+    u.vol.insert(u.vol.end(), c.vol.begin(), c.vol.end());
+    u.cc.insert(u.cc.end(), c.cc.begin(), c.cc.end());
+    u.fn.insert(u.fn.end(), c.fn.begin(), c.fn.end());
+    u.fa.insert(u.fa.end(), c.fa.begin(), c.fa.end());
+    u.fc.insert(u.fc.end(), c.fc.begin(), c.fc.end());
+    u.wdist.insert(u.wdist.end(), c.wdist.begin(), c.wdist.end());
+    shift(u.wall_faces, c.wall_faces, fo, false);
+    u.part.insert(u.part.end(), c.part.begin(), c.part.end());
+  }
+  // part lists: part p of the union = the cases' part p lists in case order
+  auto merge_parts = [&](auto get_list, auto get_off, const std::vector<int> &base, std::vector<int> &list,
+                         std::vector<int> &off) {
+    list.clear();
+    off.assign(kParts + 1, 0);
+    for (int p = 0; p < kParts; p++) {
+      for (int i = 0; i < nk; i++) {
+        const std::vector<int> &l = get_list(cases[i]);
+        const std::vector<int> &o = get_off(cases[i]);
+        for (int j = o[p]; j < o[p + 1]; j++) list.push_back(l[j] + base[i]);
+      }
+      off[p + 1] = (int)list.size();
+    }
+  };
+  merge_parts([](const HostMesh &c) -> const std::vector<int> & { return c.cell_list; },
+              [](const HostMesh &c) -> const std::vector<int> & { return c.cell_off; }, u.case_cell, u.cell_list,
+              u.cell_off);
+  merge_parts([](const HostMesh &c) -> const std::vector<int> & { return c.face_list; },
+              [](const HostMesh &c) -> const std::vector<int> & { return c.face_off; }, u.case_face, u.face_list,
+              u.face_off);
+  // waves: wave w, part p of the union = the cases' (w, p) slices in case order
+  auto merge_waves = [&](bool fwd, std::vector<int> &list, std::vector<int> &poff) {
+    int nw = 0;
+    for (const HostMesh &c : cases)
+      nw = std::max(nw, (int)(fwd ? c.fwd_poff : c.bwd_poff).size() / (kParts + 1));
+    list.clear();
+    poff.assign((size_t)nw * (kParts + 1), 0);
+    for (int w = 0; w < nw; w++) {
+      for (int p = 0; p < kParts; p++) {
+        poff[(size_t)w * (kParts + 1) + p] = (int)list.size();
+        for (int i = 0; i < nk; i++) {
+          const HostMesh &c = cases[i];
+          const std::vector<int> &l = fwd ? c.fwd_list : c.bwd_list;
+          const std::vector<int> &o = fwd ? c.fwd_poff : c.bwd_poff;
+          if ((size_t)(w + 1) * (kParts + 1) > o.size()) continue;
+          for (int j = o[(size_t)w * (kParts + 1) + p]; j < o[(size_t)w * (kParts + 1) + p + 1]; j++)
+            list.push_back(l[j] + u.case_cell[i]);
+        }
+      }
+      poff[(size_t)w * (kParts + 1) + kParts] = (int)list.size();
+    }
+  };
+  merge_waves(true, u.fwd_list, u.fwd_poff);
+  merge_waves(false, u.bwd_list, u.bwd_poff);
+  u.use_ranges = false;
 }
 
 }  // namespace usg

@@ -43,6 +43,10 @@ struct HostMesh {
   std::vector<int> face_list, face_off;       // [nf], [kParts+1]
   std::vector<int> fwd_list, fwd_poff;        // [nc], [nwaves*(kParts+1)]
   std::vector<int> bwd_list, bwd_poff;        // [nc], [nwaves*(kParts+1)]
+  // batch of independent cases (one case when built by build_mesh): case i
+  // owns cells [case_cell[i], case_cell[i+1]), faces [case_face[i],
+  // case_face[i+1]) and wall faces wall_faces[case_wall[i] .. case_wall[i+1])
+  std::vector<int> case_cell, case_face, case_wall;
 };
 
 constexpr int kParts = 8;  // XCDs of an MI355X
@@ -51,5 +55,12 @@ constexpr int kParts = 8;  // XCDs of an MI355X
 // Returns an empty string on success, else the error text.
 std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,
                        const int *flc, const int *frc, int nc);
+
+// Block-diagonal union of independently built cases, case-major: every
+// index is offset by the cases before it, wave w (and part p) of the union
+// is the concatenation of the cases' wave w (part p) lists.  Each case keeps
+// its own geometry, wall distance and numbering, so its arithmetic is
+// unchanged by batching.
+void merge_meshes(const std::vector<HostMesh> &cases, HostMesh &u);
 
 }  // namespace usg

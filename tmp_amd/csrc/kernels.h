@@ -44,6 +44,10 @@ struct DevMesh {
   // x_f - x_R, y_f - y_R, x_R - x_L, y_R - y_L), differenced in FP64 on the
   // host so that FP32 keeps their relative precision near the wall
   const float *fnxf, *fnyf, *fAf, *fdx;
+  // batch of cases: constants of case i at kc[i], case of cell c at
+  // ccase[c]; both null for a single case (the kernel argument holds them)
+  const struct Consts *kc;
+  const int *ccase;
 };
 
 // State, structure-of-arrays ([var*nc + cell]).
@@ -103,6 +107,16 @@ struct Consts {
   double fsnd_mut;     // free-stream eddy viscosity
   double sqrt_gamma;   // sqrt(1.4)
 };
+
+// Constants of the case cell c belongs to: the kernel argument for a single
+// case (B = false), the batch table otherwise.
+template <bool B>
+__device__ __forceinline__ Consts case_consts(const DevMesh &m, const Consts &k, int c) {
+  if constexpr (B)
+    return m.kc[m.ccase[c]];
+  else
+    return k;
+}
 
 #define USG_GAMMA 1.4
 

@@ -115,12 +115,13 @@ __device__ __forceinline__ void face_offset(const DevMesh &m, int cell, int f, b
 // T is the compute type (double; float in the mixed-precision mode, where
 // boundary states stay FP64 and the cell viscosity is FP64).
 // ---------------------------------------------------------------------------
-template <int FLOW, typename T>
-__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k,
+template <int FLOW, typename T, bool B>
+__global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState s, const Consts k_,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int c;
   if (!xcd_item(items, sl, c)) return;
+  const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc;
   const T *fnx = fnx_of<T>(m), *fny = fny_of<T>(m), *fA = fA_of<T>(m);
   double Ucd[4];
@@ -331,8 +332,8 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
 // Outputs per face: Fc[4], Fv[4] (viscous), Ft[2] (SST), all area-weighted,
 // oriented out of the left cell.
 // ---------------------------------------------------------------------------
-template <int FLOW, typename T>
-__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k,
+template <int FLOW, typename T, bool B>
+__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k_,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int f;
@@ -340,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   const int nc = m.nc, nf = m.nf;
   const T Gamma = T(USG_GAMMA);
   const int L = m.flc[f], Rr = m.frc[f];
+  const Consts k = case_consts<B>(m, k_, L);
   const T n0 = fnx_of<T>(m)[f], n1 = fny_of<T>(m)[f], A = fA_of<T>(m)[f];
   double ULd[4], URd[4] = {0, 0, 0, 0};
   load4(s.U, nc, L, ULd);
@@ -523,12 +525,13 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
 // ViscousFlow2D.h:223-242), D (SolveDiag, TopologyHolderStrategy.cpp:190-215),
 // the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
 // ---------------------------------------------------------------------------
-template <int FLOW, typename T>
-__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k,
+template <int FLOW, typename T, bool B>
+__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k_,
                                                 const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int c;
   if (!xcd_item(items, sl, c)) return;
+  const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc, nf = m.nf;
   int fs[4];
   int ns = 0;
@@ -641,12 +644,13 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
 // UpdatePrimitiveVariable (LUSGS_Integrator.cpp:242-255) into the second
 // state buffer, and the SST update (SSTKWStrategy.h:194-223).
 // ---------------------------------------------------------------------------
-template <int FLOW, bool FWD, typename T>
-__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k,
+template <int FLOW, bool FWD, typename T, bool B>
+__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k_,
                                                  const int *__restrict__ items, const XSlice sl) {
   using F = Flow<FLOW>;
   int c;
   if (!xcd_item(items, sl, c)) return;
+  const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc;
   const double Gamma = USG_GAMMA, OMEGAN = 1.5;
   const double diag = s.D[c];
@@ -789,10 +793,11 @@ __global__ __launch_bounds__(kBlock) void k_tcons(const DevMesh m, const DevStat
 }
 
 // Free-stream cold start (Euler2D.cpp:153-171, SSTKWStrategy.h:582-613)
-__global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const DevState s, const Consts k, int sst) {
+__global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const DevState s, const Consts k_, int sst) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = m.nc;
   if (c >= nc) return;
+  const Consts k = m.kc ? m.kc[m.ccase[c]] : k_;
   for (int j = 0; j < 4; j++) s.U[j * nc + c] = k.fs[j];
   if (sst) {
     s.tU[c] = k.fsnd_turb[0];
@@ -802,11 +807,15 @@ __global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const De
 }
 
 // ||R_rho / V||^2 partial sums, fixed-order tree per block (deterministic).
-__global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const double *R, double *partial) {
+// Density-residual norm ||R_rho / V||_2 per case.  Block b sums the squares
+// of cells [b0[b], b1[b]) (never two cases in one block); one block per case
+// then sums its blocks' partials.
+__global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const double *R, const int *b0,
+                                                        const int *b1, double *partial) {
   __shared__ double sh[kBlock];
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = b0[blockIdx.x] + threadIdx.x;
   double v = 0;
-  if (c < m.nc) {
+  if (c < b1[blockIdx.x]) {
     double r = R[c] / m.vol[c];
     v = r * r;
   }
@@ -818,51 +827,53 @@ __global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const 
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
-
-__global__ __launch_bounds__(kBlock) void k_norm_final(const double *partial, int n, double *out) {
+__device__ __forceinline__ double case_sum(const double *partial, const int *boff) {
   __shared__ double sh[kBlock];
+  const int i0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - i0;
   double v = 0;
-  for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i];
+  for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i0 + i];
   sh[threadIdx.x] = v;
   __syncthreads();
   for (int w = kBlock / 2; w > 0; w >>= 1) {
     if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sqrt(sh[0]);
+  return sh[0];
 }
-
-// Ring variant of k_norm_final for captured graphs: writes slot (*ctr % cap)
-// and advances the device counter.
-__global__ __launch_bounds__(kBlock) void k_norm_ring(const double *partial, int n, double *ring, int cap,
-                                                      int *ctr) {
-  __shared__ double sh[kBlock];
-  double v = 0;
-  for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
-    __syncthreads();
-  }
+// one block per case: out[case] = sqrt(sum)
+__global__ __launch_bounds__(kBlock) void k_norm_final(const double *partial, const int *boff, double *out) {
+  double v = case_sum(partial, boff);
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt(v);
+}
+// one block per case: ring[(ctr % cap) * ncase + case] = sqrt(sum); the
+// counter is advanced by this kernel when there is one case, else by
+// k_add_int launched after it (no block may see another's increment)
+__global__ __launch_bounds__(kBlock) void k_norm_ring(const double *partial, const int *boff, double *ring,
+                                                      int cap, int *ctr) {
+  double v = case_sum(partial, boff);
   if (threadIdx.x == 0) {
     int i = *ctr;
-    ring[i % cap] = sqrt(sh[0]);
-    *ctr = i + 1;
+    ring[(size_t)(i % cap) * gridDim.x + blockIdx.x] = sqrt(v);
+    if (gridDim.x == 1) *ctr = i + 1;
   }
 }
-
+__global__ void k_add_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
+}
 __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 // Wall force: sum over wall faces of (p n A) from the left cell's boundary state.
+// Pressure force p n A over each case's wall faces (one block per case:
+// wall faces [woff[case], woff[case+1])).
 __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const int *wall_faces,
-                                                   const int *wall_cells, int nw, double *out2) {
+                                                   const int *wall_cells, const int *woff, double *out2) {
   __shared__ double sx[kBlock], sy[kBlock];
   double fx = 0, fy = 0;
+  const int i0 = woff[blockIdx.x], nw = woff[blockIdx.x + 1] - i0;
   for (int i = threadIdx.x; i < nw; i += kBlock) {
-    int f = wall_faces[i], c = wall_cells[i];
+    int f = wall_faces[i0 + i], c = wall_cells[i0 + i];
     double p = s.U[m.nc + c];
     fx = fx + p * m.fnx[f] * m.fA[f];
     fy = fy + p * m.fny[f] * m.fA[f];
@@ -878,15 +889,10 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevSta
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out2[0] = sx[0];
-    out2[1] = sy[0];
+    out2[2 * blockIdx.x] = sx[0];
+    out2[2 * blockIdx.x + 1] = sy[0];
   }
 }
-
-// Donor-stencil apply, ROW layout (UniTioga/dataUpdate.cpp:24-72,115-133):
-// receptor value = sum_m w_m * q[donor_m], written to the receptor.
-// Reads and writes are split over two buffers' roles by the caller: donors are
-// never receptors within one apply (TIOGA's fringe/donor exclusivity).
 __global__ __launch_bounds__(kBlock) void k_stencil(double *q, int nvar, int nrecv, const int *recv, const int *nw,
                                                     const int *donor, const double *w) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -943,7 +949,8 @@ struct usg_ctx {
   int *d_cell_list = nullptr, *d_face_list = nullptr;
   int *d_wall_faces = nullptr, *d_wall_cells = nullptr;
   double *d_partial = nullptr, *d_norms = nullptr, *d_force = nullptr;
-  int nblk = 0;
+  int nblk = 0;  // blocks of a one-thread-per-cell launch
+  int nnb = 0;   // norm blocks
   int norms_cap = 0;
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -957,6 +964,13 @@ struct usg_ctx {
   hipStream_t gstream[2] = {nullptr, nullptr};
   double *d_ring = nullptr;
   int *d_ctr = nullptr;
+  // batch: cases, their parameters and constants; norm blocks (cells
+  // [nb0[b], nb1[b]) of one case each, per-case block offsets nboff)
+  int ncase = 1;
+  std::vector<usg_params> cprm;
+  std::vector<Consts> ck;
+  std::vector<double> aoa_rad;
+  int *d_nb0 = nullptr, *d_nb1 = nullptr, *d_nboff = nullptr, *d_woff = nullptr;
 };
 
 constexpr int kRing = 4096;
@@ -981,38 +995,38 @@ static int upload(usg_ctx *c, T **p, const T *src, size_t n) {
 
 // set_fs_variable2 + update_nondim_variable (TopologyHolderStrategy.cpp:325-396)
 // with WrapSolver's density 1.225 and pressure 101325 (SolverWrapper.h:55).
-static void freestream(usg_ctx *c) {
+static void freestream(const usg_params &prm, Consts &kk, double &aoa_rad) {
   const double R = 287.0, Gamma = 1.4, PI = 3.141593;
-  double fs_mach = c->prm.mach;
-  double fs_AOA = PI * c->prm.aoa_deg / 180;
-  double fs_density = 1.225, fs_pressure = 101325, fs_Re = c->prm.reynolds;
+  double fs_mach = prm.mach;
+  double fs_AOA = PI * prm.aoa_deg / 180;
+  double fs_density = 1.225, fs_pressure = 101325, fs_Re = prm.reynolds;
   double fs_soundSpeed = std::sqrt(Gamma * fs_pressure / fs_density);
   double fs_Temperature = fs_pressure / (R * fs_density);
   double fs_velocity_magnitude = fs_soundSpeed * fs_mach;
   double fs_mu = 1.458e-6 * std::pow(fs_Temperature, 1.5) / (fs_Temperature + 110.5);
   double u = fs_velocity_magnitude * std::cos(fs_AOA);
   double v = fs_velocity_magnitude * std::sin(fs_AOA);
-  c->k.fs[0] = fs_density / fs_density;
-  c->k.fs[1] = fs_pressure / fs_pressure;
-  c->k.fs[2] = u / (fs_soundSpeed / std::sqrt(Gamma));
-  c->k.fs[3] = v / (fs_soundSpeed / std::sqrt(Gamma));
-  c->k.fs_c = std::sqrt(Gamma);
-  c->k.sqrt_gamma = std::sqrt(Gamma);
-  c->k.mach = fs_mach;
-  c->k.re = fs_Re;
-  c->k.cc = 110.5 / fs_Temperature;
-  c->k.pr = 0.72;
-  c->k.prt = 0.9;
-  c->k.cfl = c->prm.cfl_lusgs;
-  c->fs_aoa_rad = fs_AOA;
+  kk.fs[0] = fs_density / fs_density;
+  kk.fs[1] = fs_pressure / fs_pressure;
+  kk.fs[2] = u / (fs_soundSpeed / std::sqrt(Gamma));
+  kk.fs[3] = v / (fs_soundSpeed / std::sqrt(Gamma));
+  kk.fs_c = std::sqrt(Gamma);
+  kk.sqrt_gamma = std::sqrt(Gamma);
+  kk.mach = fs_mach;
+  kk.re = fs_Re;
+  kk.cc = 110.5 / fs_Temperature;
+  kk.pr = 0.72;
+  kk.prt = 0.9;
+  kk.cfl = prm.cfl_lusgs;
+  aoa_rad = fs_AOA;
   // SSTkomegaModel::InitializeturbulenceVariable (SSTKWStrategy.h:582-613)
   const double L = 1.0;
   double fs_mut = fs_mu * 0.035;
-  c->k.fsnd_mut = fs_mut / fs_mu * std::sqrt(Gamma) * fs_mach / fs_Re;
+  kk.fsnd_mut = fs_mut / fs_mu * std::sqrt(Gamma) * fs_mach / fs_Re;
   double t0 = 1.5 * fs_mach * fs_soundSpeed * 0.01 * fs_mach * fs_soundSpeed * 0.01;
   double t1 = fs_density * t0 / (0.035 * fs_mu);
-  c->k.fsnd_turb[0] = Gamma * t0 / (fs_soundSpeed * fs_soundSpeed);
-  c->k.fsnd_turb[1] = std::sqrt(Gamma) * L * t1 / fs_soundSpeed;
+  kk.fsnd_turb[0] = Gamma * t0 / (fs_soundSpeed * fs_soundSpeed);
+  kk.fsnd_turb[1] = std::sqrt(Gamma) * L * t1 / fs_soundSpeed;
 }
 
 static int grid(int n) { return (n + kBlock - 1) / kBlock; }
@@ -1033,33 +1047,34 @@ static int xgrid(const XSlice &sl) {
   return kXcd * std::max(1, (mx + kBlock - 1) / kBlock);
 }
 
-template <int FLOW, typename T>
+template <int FLOW, typename T, bool B>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
-  hipLaunchKernelGGL((k_grad<FLOW, T>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
   if (ea) hipEventRecord(ea, c->stream);
-  hipLaunchKernelGGL((k_face<FLOW, T>), dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  hipLaunchKernelGGL((k_face<FLOW, T, B>), dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_face_list, fs);
   if (eb) hipEventRecord(eb, c->stream);
-  hipLaunchKernelGGL((k_cell<FLOW, T>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  hipLaunchKernelGGL((k_cell<FLOW, T, B>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
 }
 
-template <typename T>
+template <typename T, bool B>
 static void launch_residual_t(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   switch (c->prm.flow) {
-    case USG_EULER: launch_residual<USG_EULER, T>(c, ea, eb); break;
-    case USG_LAMINAR: launch_residual<USG_LAMINAR, T>(c, ea, eb); break;
-    default: launch_residual<USG_SST, T>(c, ea, eb); break;
+    case USG_EULER: launch_residual<USG_EULER, T, B>(c, ea, eb); break;
+    case USG_LAMINAR: launch_residual<USG_LAMINAR, T, B>(c, ea, eb); break;
+    default: launch_residual<USG_SST, T, B>(c, ea, eb); break;
   }
 }
 
 static void launch_residual_events(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
-  if (c->prm.precision == USG_MIXED)
-    launch_residual_t<float>(c, ea, eb);
+  const bool mixed = c->prm.precision == USG_MIXED, batch = c->ncase > 1;
+  if (mixed)
+    batch ? launch_residual_t<float, true>(c, ea, eb) : launch_residual_t<float, false>(c, ea, eb);
   else
-    launch_residual_t<double>(c, ea, eb);
+    batch ? launch_residual_t<double, true>(c, ea, eb) : launch_residual_t<double, false>(c, ea, eb);
 }
 
 // A pooled event pair for one timed launch (nullptr pair when the pool cannot grow).
@@ -1085,35 +1100,37 @@ static void launch_residual_any(usg_ctx *c) {
   launch_residual_events(c, e.first, e.second);
 }
 
-template <int FLOW, typename T>
+template <int FLOW, typename T, bool B>
 static void launch_sweeps(usg_ctx *c) {
   const HostMesh &h = c->hm;
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nw; w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, true, T>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                        c->d_fwd, sl);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nb; w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, false, T>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
                        c->d_bwd, sl);
   }
 }
 
-template <typename T>
+template <typename T, bool B>
 static void launch_sweeps_t(usg_ctx *c) {
   switch (c->prm.flow) {
-    case USG_EULER: launch_sweeps<USG_EULER, T>(c); break;
-    case USG_LAMINAR: launch_sweeps<USG_LAMINAR, T>(c); break;
-    default: launch_sweeps<USG_SST, T>(c); break;
+    case USG_EULER: launch_sweeps<USG_EULER, T, B>(c); break;
+    case USG_LAMINAR: launch_sweeps<USG_LAMINAR, T, B>(c); break;
+    default: launch_sweeps<USG_SST, T, B>(c); break;
   }
 }
 
+// per-case norms into dst[0 .. ncase)
 static void launch_norm(usg_ctx *c, double *dst) {
-  hipLaunchKernelGGL(k_norm_partial, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_partial);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, c->stream, c->d_partial, c->nblk, dst);
+  hipLaunchKernelGGL(k_norm_partial, dim3(c->nnb), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_nb0, c->d_nb1,
+                     c->d_partial);
+  hipLaunchKernelGGL(k_norm_final, dim3(c->ncase), dim3(kBlock), 0, c->stream, c->d_partial, c->d_nboff, dst);
 }
 
 static double *stbuf(usg_ctx *c) { return c->ds.U; }
@@ -1173,21 +1190,69 @@ int usg_ctx_destroy(usg_ctx *c) {
   return USG_OK;
 }
 
+static int check_params(const usg_params *params) {
+  if (params->flow < USG_EULER || params->flow > USG_SST) return fail(USG_ERR_ARG, "Undefined Flow Type");
+  if (params->precision != USG_FP64 && params->precision != USG_MIXED) return fail(USG_ERR_ARG, "bad precision");
+  return USG_OK;
+}
+
+static int ctx_setup(usg_ctx *c, usg_ctx **out);
+
 int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out) {
   if (!mesh || !params || !out) return fail(USG_ERR_ARG, "null argument");
   *out = nullptr;
-  if (params->flow < USG_EULER || params->flow > USG_SST) return fail(USG_ERR_ARG, "Undefined Flow Type");
-  if (params->precision != USG_FP64 && params->precision != USG_MIXED) return fail(USG_ERR_ARG, "bad precision");
+  int rc = check_params(params);
+  if (rc) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
   usg_ctx *c = new usg_ctx();
   c->prm = *params;
+  c->cprm = {*params};
   std::string e = build_mesh(c->hm, mesh->nn, mesh->xy, mesh->nf, mesh->fnode, mesh->flc, mesh->frc, mesh->nc);
   if (!e.empty()) {
     delete c;
     return fail(USG_ERR_MESH, e);
   }
-  freestream(c);
+  return ctx_setup(c, out);
+}
+
+int usg_batch_create(const usg_mesh_desc *meshes, const usg_params *params, int ncases, usg_ctx **out) {
+  if (!meshes || !params || !out || ncases <= 0) return fail(USG_ERR_ARG, "null argument or no cases");
+  *out = nullptr;
+  for (int i = 0; i < ncases; i++) {
+    int rc = check_params(&params[i]);
+    if (rc) return rc;
+    if (params[i].flow != params[0].flow || params[i].precision != params[0].precision ||
+        params[i].cfl_lusgs != params[0].cfl_lusgs || params[i].cfl_rk != params[0].cfl_rk)
+      return fail(USG_ERR_ARG, "cases of a batch share flow type, precision and CFL numbers");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
+  std::vector<HostMesh> cases(ncases);
+  long total = 0;
+  for (int i = 0; i < ncases; i++) {
+    const usg_mesh_desc &d = meshes[i];
+    std::string e = build_mesh(cases[i], d.nn, d.xy, d.nf, d.fnode, d.flc, d.frc, d.nc);
+    if (!e.empty()) return fail(USG_ERR_MESH, "case " + std::to_string(i) + ": " + e);
+    total += (long)d.nc + d.nf;
+    if (total > 2000000000L) return fail(USG_ERR_ARG, "batch exceeds 32-bit cell/face indexing");
+  }
+  usg_ctx *c = new usg_ctx();
+  c->prm = params[0];
+  c->cprm.assign(params, params + ncases);
+  merge_meshes(cases, c->hm);
+  return ctx_setup(c, out);
+}
+
+// Device buffers and constants of a context whose host mesh and parameters are set.
+static int ctx_setup(usg_ctx *c, usg_ctx **out) {
+  const usg_params *params = &c->prm;
+  c->ncase = (int)c->cprm.size();
+  c->ck.resize(c->ncase);
+  c->aoa_rad.resize(c->ncase);
+  for (int i = 0; i < c->ncase; i++) freestream(c->cprm[i], c->ck[i], c->aoa_rad[i]);
+  c->k = c->ck[0];
+  c->fs_aoa_rad = c->aoa_rad[0];
   int rc = USG_OK;
 #define CK(x)                  \
   do {                         \
@@ -1325,10 +1390,33 @@ int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx 
     CK(dalloc(c, &s.tdW, 2 * (size_t)nc));
     CK(dalloc(c, &s.tD, 2 * (size_t)nc));
   }
-  CK(dalloc(c, &c->d_partial, c->nblk));
-  CK(dalloc(c, &c->d_force, 2));
-  CK(ensure_norms(c, 16));
-  CK(dalloc(c, &c->d_ring, kRing));
+  {
+    // norm blocks: kBlock cells of one case each; per-case block offsets
+    std::vector<int> nb0, nb1, nboff(1, 0);
+    for (int i = 0; i < c->ncase; i++) {
+      for (int a = h.case_cell[i]; a < h.case_cell[i + 1]; a += kBlock) {
+        nb0.push_back(a);
+        nb1.push_back(std::min(a + kBlock, h.case_cell[i + 1]));
+      }
+      nboff.push_back((int)nb0.size());
+    }
+    c->nnb = (int)nb0.size();
+    CK(upload(c, &c->d_nb0, nb0.data(), nb0.size()));
+    CK(upload(c, &c->d_nb1, nb1.data(), nb1.size()));
+    CK(upload(c, &c->d_nboff, nboff.data(), nboff.size()));
+    CK(upload(c, &c->d_woff, h.case_wall.data(), h.case_wall.size()));
+    if (c->ncase > 1) {
+      std::vector<int> ccase(nc);
+      for (int i = 0; i < c->ncase; i++)
+        for (int a = h.case_cell[i]; a < h.case_cell[i + 1]; a++) ccase[a] = i;
+      CK(upload(c, (int **)&dm.ccase, ccase.data(), ccase.size()));
+      CK(upload(c, (Consts **)&dm.kc, c->ck.data(), c->ck.size()));
+    }
+  }
+  CK(dalloc(c, &c->d_partial, c->nnb));
+  CK(dalloc(c, &c->d_force, 2 * (size_t)c->ncase));
+  CK(ensure_norms(c, 16 * c->ncase));
+  CK(dalloc(c, &c->d_ring, (size_t)kRing * c->ncase));
   CK(dalloc(c, &c->d_ctr, 1));
 #undef CK
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
@@ -1393,16 +1481,19 @@ int usg_residual(usg_ctx *c) {
 }
 
 static void launch_sweeps_any(usg_ctx *c) {
-  if (c->prm.precision == USG_MIXED)
-    launch_sweeps_t<float>(c);
+  const bool mixed = c->prm.precision == USG_MIXED, batch = c->ncase > 1;
+  if (mixed)
+    batch ? launch_sweeps_t<float, true>(c) : launch_sweeps_t<float, false>(c);
   else
-    launch_sweeps_t<double>(c);
+    batch ? launch_sweeps_t<double, true>(c) : launch_sweeps_t<double, false>(c);
 }
 
 static void launch_norm_ring(usg_ctx *c) {
-  hipLaunchKernelGGL(k_norm_partial, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_partial);
-  hipLaunchKernelGGL(k_norm_ring, dim3(1), dim3(kBlock), 0, c->stream, c->d_partial, c->nblk, c->d_ring, kRing,
-                     c->d_ctr);
+  hipLaunchKernelGGL(k_norm_partial, dim3(c->nnb), dim3(kBlock), 0, c->stream, c->dm, c->ds.R, c->d_nb0, c->d_nb1,
+                     c->d_partial);
+  hipLaunchKernelGGL(k_norm_ring, dim3(c->ncase), dim3(kBlock), 0, c->stream, c->d_partial, c->d_nboff, c->d_ring,
+                     kRing, c->d_ctr);
+  if (c->ncase > 1) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c->stream, c->d_ctr, 1);
 }
 
 // One LU-SGS iteration (LUSGSIntegrator::UpdateSerial): residual, norm into
@@ -1489,7 +1580,8 @@ int usg_lusgs_step(usg_ctx *c, int n, double *res_hist) {
     }
     HIPCHK(hipGetLastError());
     if (res_hist)
-      HIPCHK(hipMemcpyAsync(res_hist + done, c->d_ring, chunk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(res_hist + (size_t)done * c->ncase, c->d_ring, (size_t)chunk * c->ncase * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c->stream, c->d_ctr, 0);
     done += chunk;
   }
@@ -1501,7 +1593,7 @@ int usg_rk5_step(usg_ctx *c, int n, double *res_hist) {
   if (!c || n < 0) return fail(USG_ERR_ARG, "bad argument");
   if (c->prm.flow == USG_SST) return fail(USG_ERR_ARG, "RK5 is provided for Euler and laminar flow");
   if (res_hist) {
-    int rc = ensure_norms(c, n);
+    int rc = ensure_norms(c, n * c->ncase);
     if (rc) return rc;
   }
   const double alpha[5] = {1.0 / 4, 1.0 / 6, 3.0 / 8, 1.0 / 2, 1.0};
@@ -1509,14 +1601,15 @@ int usg_rk5_step(usg_ctx *c, int n, double *res_hist) {
     hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W0);
     for (int st = 0; st < 5; st++) {
       launch_residual_any(c);
-      if (res_hist && st == 0) launch_norm(c, c->d_norms + it);
+      if (res_hist && st == 0) launch_norm(c, c->d_norms + (size_t)it * c->ncase);
       hipLaunchKernelGGL(k_rk_stage, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->prm.cfl_rk,
                          alpha[st], st, 0);
     }
   }
   HIPCHK(hipGetLastError());
   if (res_hist && n > 0)
-    HIPCHK(hipMemcpyAsync(res_hist, c->d_norms, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(res_hist, c->d_norms, (size_t)n * c->ncase * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
   c->have_residual = false;
   return check_err(c);
 }
@@ -1575,26 +1668,35 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
 
 int usg_res_l2(usg_ctx *c, double *out) {
   if (!c || !out) return fail(USG_ERR_ARG, "null argument");
-  int rc = ensure_norms(c, 1);
+  int rc = ensure_norms(c, c->ncase);
   if (rc) return rc;
   launch_norm(c, c->d_norms);
-  HIPCHK(hipMemcpyAsync(out, c->d_norms, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(out, c->d_norms, c->ncase * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return USG_OK;
 }
 
 int usg_forces(usg_ctx *c, double *cl, double *cd) {
   if (!c || !cl || !cd) return fail(USG_ERR_ARG, "null argument");
-  int nw = (int)c->hm.wall_faces.size();
-  hipLaunchKernelGGL(k_forces, dim3(1), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->d_wall_faces,
-                     c->d_wall_cells, nw, c->d_force);
-  double f[2];
-  HIPCHK(hipMemcpyAsync(f, c->d_force, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_forces, dim3(c->ncase), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->d_wall_faces,
+                     c->d_wall_cells, c->d_woff, c->d_force);
+  std::vector<double> f(2 * (size_t)c->ncase);
+  HIPCHK(hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  double q = 0.5 * 1.4 * c->prm.mach * c->prm.mach;
-  double ca = std::cos(c->fs_aoa_rad), sa = std::sin(c->fs_aoa_rad);
-  *cl = (-f[0] * sa + f[1] * ca) / q;
-  *cd = (f[0] * ca + f[1] * sa) / q;
+  for (int i = 0; i < c->ncase; i++) {
+    double q = 0.5 * 1.4 * c->cprm[i].mach * c->cprm[i].mach;
+    double ca = std::cos(c->aoa_rad[i]), sa = std::sin(c->aoa_rad[i]);
+    cl[i] = (-f[2 * i] * sa + f[2 * i + 1] * ca) / q;
+    cd[i] = (f[2 * i] * ca + f[2 * i + 1] * sa) / q;
+  }
+  return USG_OK;
+}
+
+int usg_case_count(usg_ctx *c, int *ncases, int *cell_off) {
+  if (!c || !ncases) return fail(USG_ERR_ARG, "null argument");
+  *ncases = c->ncase;
+  if (cell_off)
+    for (int i = 0; i <= c->ncase; i++) cell_off[i] = c->hm.case_cell[i];
   return USG_OK;
 }
 

@@ -51,7 +51,8 @@ class _Params(C.Structure):
 EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "usg_init_freestream",
            "usg_set_prim", "usg_residual", "usg_lusgs_step", "usg_rk5_step", "usg_get_field", "usg_res_l2",
            "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
-           "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error"]
+           "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
+           "usg_batch_create", "usg_case_count"]
 
 _lib = None
 
@@ -84,6 +85,8 @@ def lib():
         L.usg_stencil_apply.argtypes = [vp, C.c_int, C.c_int, _ip, _ip, _ip, _dp, vp]
         L.usg_copy_kernel.argtypes = [vp, vp, C.c_long, vp]
         L.usg_use_graphs.argtypes = [vp, C.c_int]
+        L.usg_batch_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_Params), C.c_int, C.POINTER(vp)]
+        L.usg_case_count.argtypes = [vp, _ip, _ip]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -122,6 +125,8 @@ class Solver:
         self.mesh = m
         self.nc = m.nc
         self.flow = flow
+        self.ncase = 1
+        self.case_cell = np.array([0, m.nc])
         d, keep = _desc(m)
         p = _Params(flow, precision, mach, aoa, reynolds, cfl_lusgs, cfl_rk)
         h = C.c_void_p()
@@ -175,25 +180,32 @@ class Solver:
     def residual(self):
         _check(lib().usg_residual(self._h))
 
+    def _steps(self, fn, n, history):
+        h = np.zeros(max(n, 1) * self.ncase) if history else None
+        _check(fn(self._h, n, h.ctypes.data_as(_dp) if history else None))
+        if not history:
+            return None
+        h = h[:n * self.ncase].reshape(n, self.ncase)
+        return h if isinstance(self, BatchSolver) else h[:, 0]
+
     def lusgs(self, n: int = 1, history: bool = False):
-        h = np.zeros(max(n, 1)) if history else None
-        _check(lib().usg_lusgs_step(self._h, n, h.ctypes.data_as(_dp) if history else None))
-        return h[:n] if history else None
+        """n LU-SGS iterations; with history, ||R_rho/V||_2 per iteration."""
+        return self._steps(lib().usg_lusgs_step, n, history)
 
     def rk5(self, n: int = 1, history: bool = False):
-        h = np.zeros(max(n, 1)) if history else None
-        _check(lib().usg_rk5_step(self._h, n, h.ctypes.data_as(_dp) if history else None))
-        return h[:n] if history else None
+        return self._steps(lib().usg_rk5_step, n, history)
 
-    def res_l2(self) -> float:
-        v = C.c_double()
-        _check(lib().usg_res_l2(self._h, C.byref(v)))
-        return v.value
+    def res_l2(self):
+        v = np.zeros(self.ncase)
+        _check(lib().usg_res_l2(self._h, v.ctypes.data_as(_dp)))
+        return v if isinstance(self, BatchSolver) else float(v[0])
 
     def forces(self):
-        cl, cd = C.c_double(), C.c_double()
-        _check(lib().usg_forces(self._h, C.byref(cl), C.byref(cd)))
-        return cl.value, cd.value
+        cl, cd = np.zeros(self.ncase), np.zeros(self.ncase)
+        _check(lib().usg_forces(self._h, cl.ctypes.data_as(_dp), cd.ctypes.data_as(_dp)))
+        if isinstance(self, BatchSolver):
+            return cl, cd
+        return float(cl[0]), float(cd[0])
 
     def waves(self):
         a, b = C.c_int(), C.c_int()
@@ -213,3 +225,36 @@ class Solver:
         t, n = C.c_double(), C.c_long()
         _check(lib().usg_kernel_time(self._h, C.byref(t), C.byref(n)))
         return t.value, n.value
+
+
+class BatchSolver(Solver):
+    """Many independent cases in one context (usg_batch_create): per-case
+    free streams, one launch sequence per iteration for all of them.  Fields
+    are case-major over the union; per-case outputs are arrays."""
+
+    def __init__(self, meshes, flow: int = EULER, mach=0.5, aoa=0.0, reynolds=1e6, precision: int = FP64,
+                 cfl_lusgs: float = 400.0, cfl_rk: float = 0.8):
+        k = len(meshes)
+        mach, aoa, reynolds = (np.broadcast_to(np.asarray(v, np.float64), (k,)) for v in (mach, aoa, reynolds))
+        descs = (_MeshDesc * k)()
+        self._keep = []
+        for i, m in enumerate(meshes):
+            d, keep = _desc(m)
+            descs[i] = d
+            self._keep.append(keep)
+        params = (_Params * k)(*[_Params(flow, precision, float(mach[i]), float(aoa[i]), float(reynolds[i]),
+                                         cfl_lusgs, cfl_rk) for i in range(k)])
+        h = C.c_void_p()
+        _check(lib().usg_batch_create(descs, params, k, C.byref(h)))
+        self._h = h
+        self.meshes = list(meshes)
+        self.flow = flow
+        n = C.c_int()
+        off = np.zeros(k + 1, np.int32)
+        _check(lib().usg_case_count(h, C.byref(n), off.ctypes.data_as(_ip)))
+        self.ncase = n.value
+        self.case_cell = off.astype(np.int64)
+        self.nc = int(off[-1])
+
+    def case_slice(self, i: int) -> slice:
+        return slice(int(self.case_cell[i]), int(self.case_cell[i + 1]))
