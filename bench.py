@@ -23,6 +23,8 @@ WORKLOADS = {
     "sst1m": ("airfoil_1m", 2, 0.15, 4.0, 6e6, "SST k-omega RANS FP64, NACA0012 hybrid ~1M cells"),
     "euler1m": ("airfoil_1m", 0, 0.15, 4.0, 6e6, "Euler FP64, NACA0012 hybrid ~1M cells"),
     "bump20k": ("bump", 0, 0.5, 0.0, 1e6, "Euler FP64, channel bump ~20k cells"),
+    # 64 NACA 4-digit cases (seed 4004), per-case Mach and alpha, one batched context
+    "batch64": ("batch", 2, None, None, 6e6, "SST k-omega RANS FP64, 64-case NACA 4-digit batch (seed 4004)"),
 }
 
 
@@ -35,6 +37,21 @@ def build_mesh(kind):
     else:
         raise ValueError(kind)
     return mesh.prepare(m)
+
+
+def _batch_case_mesh(case):
+    from tmp_amd import mesh
+    return mesh.prepare(mesh.batch_mesh(case))[0]
+
+
+def build_batch(workers=8):
+    """The 64 batch cases and their prepared meshes (built in a process pool)."""
+    from concurrent.futures import ProcessPoolExecutor
+    from tmp_amd import mesh
+    cases = mesh.batch_cases()
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        meshes = list(ex.map(_batch_case_mesh, cases))
+    return cases, meshes
 
 
 def cpu_baseline(m, flow, mach, aoa, re, seconds):
@@ -54,6 +71,23 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
             break
     dt = time.perf_counter() - t0
     return m.nc * n / dt, n, dt
+
+
+def _cpu_case(args):
+    case, flow, re, seconds = args
+    m = _batch_case_mesh(case)
+    v, n, dt = cpu_baseline(m, flow, case["mach"], case["aoa"], re, seconds)
+    return v, n, m.nc
+
+
+def cpu_baseline_batch(cases, flow, re, seconds, workers):
+    """One batch case per core (the oracle, 1 thread each) on ``workers``
+    cores at once, each for ``seconds``: summed cell-iterations/s."""
+    from concurrent.futures import ProcessPoolExecutor
+    sample = cases[:workers]
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        res = list(ex.map(_cpu_case, [(c, flow, re, seconds) for c in sample]))
+    return sum(r[0] for r in res), sum(r[1] for r in res), len(sample)
 
 
 def measured_traffic(workload, nc):
@@ -104,6 +138,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed: FP32 gradients/limiter/fluxes, FP64 state, accumulation, diagonal and norms")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick threads per block per kernel group (cached in ~/.cache/tmp_amd/autotune.json)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (traffic-counter calibration)")
     a = ap.parse_args()
@@ -117,13 +153,40 @@ def main():
     from tmp_amd import roofline, usg
 
     kind, flow, mach, aoa, re, label = WORKLOADS[a.workload]
-    t = time.perf_counter()
-    m, ncol = build_mesh(kind)
-    print(f"# mesh {m.name}: {m.nc} cells, {m.nf} faces, {ncol} colours, built in {time.perf_counter() - t:.1f}s",
-          file=sys.stderr, flush=True)
     mixed = a.precision == "mixed"
-    solver = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re, precision=usg.MIXED if mixed else usg.FP64)
+    prec = usg.MIXED if mixed else usg.FP64
+    t = time.perf_counter()
+    if kind == "batch":
+        cases, meshes = build_batch()
+        nc, nf = sum(m.nc for m in meshes), sum(m.nf for m in meshes)
+        print(f"# batch of {len(meshes)} cases: {nc} cells, {nf} faces, built in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
+        solver = usg.BatchSolver(meshes, flow=flow, mach=[c["mach"] for c in cases],
+                                 aoa=[c["aoa"] for c in cases], reynolds=re, precision=prec)
+        config = {"workload": label, "cases": len(meshes), "cells": nc, "faces": nf,
+                  "case_cells": [min(m.nc for m in meshes), max(m.nc for m in meshes)],
+                  "mach": [0.3, 0.8], "aoa_deg": [-4.0, 8.0], "reynolds": re}
+        metric = "cell-iters/s, 64-case SST batch"
+    else:
+        m, ncol = build_mesh(kind)
+        nc, nf = m.nc, m.nf
+        print(f"# mesh {m.name}: {nc} cells, {nf} faces, {ncol} colours, built in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
+        solver = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re, precision=prec)
+        config = {"workload": label, "cells": nc, "faces": nf, "colours": ncol, "mach": mach, "aoa_deg": aoa,
+                  "reynolds": re}
+        metric = "cell-iters/s at 1M-cell SST"
+    if mixed:
+        config["workload"] = config["workload"].replace("FP64", "mixed FP32/FP64")
     nfw, nbw = solver.waves()
+    config.update({"lusgs_waves": [nfw, nbw], "integrator": "LU-SGS CFL 400", "parallelism": "single GPU"})
+    if a.autotune:
+        from tmp_amd import autotune
+        t = time.perf_counter()
+        blocks, hit = autotune.tune(solver, prec, nf)
+        print(f"# autotune {'cache hit' if hit else 'search'}: blocks {blocks} ({time.perf_counter() - t:.1f}s)",
+              file=sys.stderr, flush=True)
+    config["threads_per_block"] = solver.blocks()
     if a.warmup:
         solver.lusgs(a.warmup)
     solver.sync()
@@ -140,17 +203,23 @@ def main():
     t3 = time.perf_counter()
     ktime_ms, klaunch = solver.kernel_time()
     solver.timing(False)
-    value = m.nc * a.steps / elapsed
-    r = m.nf / m.nc
-    kbytes = roofline.face_kernel_bytes_per_cell(flow, r, mixed) * m.nc
+    if kind == "batch":
+        st = solver.status()
+        config["failed_cases"] = int((st != 0).sum())
+        print(f"# cases frozen by a failed check: {config['failed_cases']} of {len(st)}", file=sys.stderr,
+              flush=True)
+    value = nc * a.steps / elapsed
+    r = nf / nc
+    kbytes = roofline.face_kernel_bytes_per_cell(flow, r, mixed) * nc
     kavg_s = ktime_ms / 1e3 / max(klaunch, 1)
     achieved = kbytes / kavg_s / 1e9
     it_frac = roofline.iteration_bytes_per_cell(flow, r) * value / 1e9 / roofline.HBM_PEAK_GBS
-    print(f"# residual history {hist[0]:.4e} -> {hist[-1]:.4e}; flux kernel avg {kavg_s * 1e6:.1f} us over "
-          f"{klaunch} launches; iteration model {it_frac:.3f} of HBM peak; eager "
+    h = hist.reshape(a.steps, -1)
+    print(f"# residual history (case 0) {h[0, 0]:.4e} -> {h[-1, 0]:.4e}; flux kernel avg {kavg_s * 1e6:.1f} us "
+          f"over {klaunch} launches; iteration model {it_frac:.3f} of HBM peak; eager "
           f"{(t3 - t2) / a.steps * 1e3:.3f} ms/step", file=sys.stderr, flush=True)
     out = {
-        "metric": "cell-iters/s at 1M-cell SST",
+        "metric": metric,
         "value": value,
         "unit": "cell-iterations/s",
         "n_gpus": 1,
@@ -162,22 +231,26 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic: generated mesh, free-stream cold start",
-        "config": {"workload": label.replace("FP64", "mixed FP32/FP64") if mixed else label, "cells": m.nc, "faces": m.nf, "colours": ncol,
-                   "lusgs_waves": [nfw, nbw], "mach": mach, "aoa_deg": aoa, "reynolds": re,
-                   "integrator": "LU-SGS CFL 400", "parallelism": "single GPU"},
+        "config": config,
         "roofline": {"bound": "hbm", "kernel": "k_face", "achieved": achieved,
                      "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / roofline.HBM_PEAK_GBS,
-                     "traffic": None if mixed else measured_traffic(a.workload, m.nc),
+                     "traffic": None if mixed else measured_traffic(a.workload, nc),
                      "iteration_frac": it_frac},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        v, n, dt = cpu_baseline(m, flow, mach, aoa, re, a.cpu_seconds)
-        out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": 1, "kind": "port",
-                               "sample": f"{n} LU-SGS iterations of the same {m.nc}-cell workload from the "
-                                         f"free stream ({dt:.1f}s, oracle C restatement, -O2, 1 thread)"}
+        if kind == "batch":
+            workers = min(16, os.cpu_count() or 1)
+            v, n, k = cpu_baseline_batch(cases, flow, re, a.cpu_seconds, workers)
+            out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": workers, "kind": "port",
+                                   "sample": f"first {k} batch cases, one per core, {n} LU-SGS iterations in "
+                                             f"total over {a.cpu_seconds:.0f}s each (oracle C restatement, -O2)"}
+        else:
+            v, n, dt = cpu_baseline(m, flow, mach, aoa, re, a.cpu_seconds)
+            out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": 1, "kind": "port",
+                                   "sample": f"{n} LU-SGS iterations of the same {nc}-cell workload from the "
+                                             f"free stream ({dt:.1f}s, oracle C restatement, -O2, 1 thread)"}
     print(json.dumps(out), flush=True)
-
 
 if __name__ == "__main__":
     main()

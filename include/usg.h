@@ -99,6 +99,13 @@ int usg_batch_create(const usg_mesh_desc *meshes, const usg_params *params, int 
 /* Number of cases and, if cell_off is not NULL, their cell ranges
  * (cell_off[ncases+1]: case i owns cells [cell_off[i], cell_off[i+1])). */
 int usg_case_count(usg_ctx *ctx, int *ncases, int *cell_off);
+/* Per-case status of a batch, status[ncases]: 0 = running, bit 0 = the HLLC
+ * p* check failed (HLLCFluxStrategy.cpp:77-86), bit 1 = the slip-wall check
+ * failed (BoundaryStrategy.h:250-256).  Where a single context's call fails
+ * with USG_ERR_NUMERIC, a batch freezes the failed case at its state of that
+ * iteration and carries on with the others; usg_set_prim and
+ * usg_init_freestream clear the status. */
+int usg_case_status(usg_ctx *ctx, int *status);
 int usg_ctx_destroy(usg_ctx *ctx);
 /* hipStream_t to launch on (NULL = the context's own stream). */
 int usg_set_stream(usg_ctx *ctx, void *hip_stream);
@@ -147,6 +154,24 @@ int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *f
 /* LU-SGS iterations run as replays of a captured two-iteration hipGraph
  * (default on); off = eager launches.  Results are identical either way. */
 int usg_use_graphs(usg_ctx *ctx, int on);
+
+/* Threads per block of each kernel group (64, 128 or 256; default 256):
+ * the gradient/limiter kernel, the face flux kernel, the cell gather kernel
+ * and the sweep waves.  No counterpart in the reference (a GPU launch
+ * parameter). */
+#define USG_K_GRAD 0
+#define USG_K_FACE 1
+#define USG_K_CELL 2
+#define USG_K_SWEEP 3
+int usg_set_block(usg_ctx *ctx, int group, int threads);
+int usg_get_block(usg_ctx *ctx, int group, int *threads);
+/* Autotuner: for each group in turn, times `iters` eager LU-SGS iterations
+ * with 64, 128 and 256 threads per block and keeps the fastest; the state
+ * is restored afterwards, so tuning does not change the solution.  best[4]
+ * (may be NULL) receives the chosen sizes.  No counterpart in the reference. */
+int usg_autotune(usg_ctx *ctx, int iters, int *best);
+/* gcnArchName of the current device (key of autotuning caches). */
+int usg_device_arch(char *buf, int n);
 
 /* Kernel timing: when enabled, every launch of the dominant kernel (the face
  * flux kernel) is bracketed by HIP events on its stream; iterations then run

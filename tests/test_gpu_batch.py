@@ -68,3 +68,47 @@ def test_batch_rk5_equals_single_cases():
     hs = [s.rk5(4, history=True) for s in singles]
     _check_cases(b, singles, hb, hs)
 
+
+
+def test_autotune_restores_state_and_blocks_do_not_change_results():
+    """Tuning leaves the state as it found it, and any block sizes give the
+    same bits (one thread per cell or face whatever the block size)."""
+    m = airfoil_small()
+    a = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
+    b = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
+    a.lusgs(3)
+    b.lusgs(3)
+    best = a.autotune(2)
+    assert all(t in (64, 128, 256) for t in best) and a.blocks() == best
+    assert np.array_equal(a.U, b.U)
+    a.set_blocks([64, 128, 64, 128])
+    ha = a.lusgs(5, history=True)
+    hb = b.lusgs(5, history=True)
+    assert np.array_equal(ha, hb) and np.array_equal(a.U, b.U)
+    assert np.array_equal(a.field(usg.F_TURB), b.field(usg.F_TURB))
+
+
+def test_failed_case_is_frozen_and_reported():
+    """A case whose residual fails the HLLC p* check (a negative pressure
+    makes the sound speed NaN) is frozen at its state and reported; the
+    other cases run on unchanged.  A single context fails the call instead,
+    with the reference's message."""
+    m = part2()
+    b = usg.BatchSolver([m, m], flow=usg.EULER, mach=0.5, aoa=2.0)
+    U = b.U.copy()
+    bad = U[b.case_slice(1)]
+    bad[0, 1] = -1.0                      # pressure of one cell of case 1
+    U[b.case_slice(1)] = bad
+    b.set_prim(U)
+    b.lusgs(4)
+    assert b.status().tolist() == [0, 1]
+    assert np.array_equal(b.U[b.case_slice(1)], bad)
+    s = usg.Solver(m, flow=usg.EULER, mach=0.5, aoa=2.0)
+    s.lusgs(4)
+    assert np.array_equal(b.U[b.case_slice(0)], s.U)
+    one = usg.Solver(m, flow=usg.EULER, mach=0.5, aoa=2.0)
+    one.set_prim(bad)
+    with pytest.raises(usg.UsgError, match="p star"):
+        one.lusgs(1)
+    b.set_prim(b.U * 0 + s.U[0])          # a fresh state clears the status
+    assert b.status().tolist() == [0, 0]

@@ -261,7 +261,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
     muc = sutherland<double>(k, Tc);
     s.mu[c] = muc;
   }
-  if (bad) atomicAdd(&s.err[1], 1);
+  if (bad) {
+    atomicAdd(&s.err[1], 1);
+    if constexpr (B) atomicOr(&s.err[2 + m.ccase[c]], 2);
+  }
 
   // Venkatakrishnan limiter: eps^2 = 5^3 * V_left (integer 3/ndim == 1)
   T d1max[4], d1min[4], ph[4] = {1.0, 1.0, 1.0, 1.0};
@@ -354,7 +357,10 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
     T Ua[4], Ub[4];
     reconstruct<T>(m, s, L, f, true, UL, Ua);
     reconstruct<T>(m, s, Rr, f, false, UR, Ub);
-    if (!hllc(Ua, Ub, n0, n1, A, Fx)) atomicAdd(&s.err[0], 1);
+    if (!hllc(Ua, Ub, n0, n1, A, Fx)) {
+      atomicAdd(&s.err[0], 1);
+      if constexpr (B) atomicOr(&s.err[2 + m.ccase[L]], 1);
+    }
   } else {
     // boundary faces in FP64 in every mode
     double Ued[4], Fxd[4];
@@ -655,6 +661,26 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
   const double Gamma = USG_GAMMA, OMEGAN = 1.5;
   const double diag = s.D[c];
   const double mprc = F::visc ? s.mprc[c] : 0.0;
+  if constexpr (B && !FWD) {
+    // a batch case whose residual failed a check in this iteration keeps
+    // its state (the other cases are independent of it)
+    if (s.err[2 + m.ccase[c]]) {
+      for (int j = 0; j < 4; j++) {
+        s.dW[j * nc + c] = 0.0;
+        s.Wn[j * nc + c] = s.W[j * nc + c];
+        s.Un[j * nc + c] = s.U[j * nc + c];
+      }
+      if (F::sst) {
+        for (int q = 0; q < 2; q++) {
+          s.tdW[q * nc + c] = 0.0;
+          s.tWn[q * nc + c] = s.tW[q * nc + c];
+          s.tUn[q * nc + c] = s.tU[q * nc + c];
+        }
+        s.mutn[c] = s.mut[c];
+      }
+      return;
+    }
+  }
   const double *dsrc = FWD ? s.dW1 : s.dW;
   const double *tsrc = FWD ? s.tdW1 : s.tdW;
   double dFi[4] = {0, 0, 0, 0};
@@ -971,6 +997,8 @@ struct usg_ctx {
   std::vector<Consts> ck;
   std::vector<double> aoa_rad;
   int *d_nb0 = nullptr, *d_nb1 = nullptr, *d_nboff = nullptr, *d_woff = nullptr;
+  // threads per block of the kernel groups (USG_K_GRAD .. USG_K_SWEEP)
+  int blk[4] = {kBlock, kBlock, kBlock, kBlock};
 };
 
 constexpr int kRing = 4096;
@@ -1040,23 +1068,24 @@ static XSlice slice(const int *off) {
   for (int p = 0; p <= kXcd; p++) sl.off[p] = off[p];
   return sl;
 }
-// grid of a part-split launch: kXcd x (blocks of the largest part)
-static int xgrid(const XSlice &sl) {
+// grid of a part-split launch of bs-thread blocks: kXcd x (blocks of the largest part)
+static int xgrid(const XSlice &sl, int bs) {
   int mx = 0;
   for (int p = 0; p < kXcd; p++) mx = std::max(mx, sl.off[p + 1] - sl.off[p]);
-  return kXcd * std::max(1, (mx + kBlock - 1) / kBlock);
+  return kXcd * std::max(1, (mx + bs - 1) / bs);
 }
 
 template <int FLOW, typename T, bool B>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
-  hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  const int bg = c->blk[USG_K_GRAD], bf = c->blk[USG_K_FACE], bc = c->blk[USG_K_CELL];
+  hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs, bg)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
   if (ea) hipEventRecord(ea, c->stream);
-  hipLaunchKernelGGL((k_face<FLOW, T, B>), dim3(xgrid(fs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  hipLaunchKernelGGL((k_face<FLOW, T, B>), dim3(xgrid(fs, bf)), dim3(bf), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_face_list, fs);
   if (eb) hipEventRecord(eb, c->stream);
-  hipLaunchKernelGGL((k_cell<FLOW, T, B>), dim3(xgrid(cs)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
+  hipLaunchKernelGGL((k_cell<FLOW, T, B>), dim3(xgrid(cs, bc)), dim3(bc), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
 }
 
@@ -1103,17 +1132,18 @@ static void launch_residual_any(usg_ctx *c) {
 template <int FLOW, typename T, bool B>
 static void launch_sweeps(usg_ctx *c) {
   const HostMesh &h = c->hm;
+  const int bs = c->blk[USG_K_SWEEP];
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nw; w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_fwd, sl);
+    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
+                       c->k, c->d_fwd, sl);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
   for (int w = 0; w < nb; w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_bwd, sl);
+    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
+                       c->k, c->d_bwd, sl);
   }
 }
 
@@ -1158,15 +1188,27 @@ static int ensure_norms(usg_ctx *c, int n) {
   return USG_OK;
 }
 
+// Invariant checks of the last launches.  A single case fails the call with
+// the reference's message.  In a batch, failed cases are frozen by the
+// sweep kernels and reported through usg_case_status; the call succeeds.
 static int check_err(usg_ctx *c) {
-  int h[2] = {0, 0};
-  HIPCHK(hipMemcpyAsync(h, c->ds.err, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> h(2 + (c->ncase > 1 ? c->ncase : 0), 0);
+  HIPCHK(hipMemcpyAsync(h.data(), c->ds.err, h.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (h[0] || h[1]) {
-    HIPCHK(hipMemsetAsync(c->ds.err, 0, sizeof(h), c->stream));
+    HIPCHK(hipMemsetAsync(c->ds.err, 0, 2 * sizeof(int), c->stream));
+    if (c->ncase > 1) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      return USG_OK;
+    }
     return fail(USG_ERR_NUMERIC, h[0] ? "SOmething wrong at p star (HLLC |p*_L - p*_R| >= 1e-10)"
                                       : "Something wrong with Vector computation (slip wall |t.n| > 1e-7)");
   }
+  return USG_OK;
+}
+
+static int clear_case_status(usg_ctx *c) {
+  if (c->ncase > 1) HIPCHK(hipMemsetAsync(c->ds.err + 2, 0, c->ncase * sizeof(int), c->stream));
   return USG_OK;
 }
 
@@ -1359,7 +1401,7 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(dalloc(c, &s.dt_rk, nc));
   CK(dalloc(c, &s.W0, 4 * (size_t)nc));
   CK(dalloc(c, &s.D, nc));
-  CK(dalloc(c, &s.err, 2));
+  CK(dalloc(c, &s.err, 2 + (c->ncase > 1 ? (size_t)c->ncase : 0)));
   if (visc) {
     if (mixed)
       CK(dalloc(c, &s.gradTf, 2 * (size_t)nc));
@@ -1441,6 +1483,8 @@ int usg_sync(usg_ctx *c) {
 
 int usg_init_freestream(usg_ctx *c) {
   if (!c) return fail(USG_ERR_ARG, "null ctx");
+  int rc = clear_case_status(c);
+  if (rc) return rc;
   const bool sst = c->prm.flow == USG_SST;
   hipLaunchKernelGGL(k_freestream, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->k, (int)sst);
   hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
@@ -1453,6 +1497,8 @@ int usg_init_freestream(usg_ctx *c) {
 
 int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
   if (!c || !U) return fail(USG_ERR_ARG, "null argument");
+  int rc = clear_case_status(c);
+  if (rc) return rc;
   const int nc = c->hm.nc;
   std::vector<double> soa(4 * (size_t)nc);
   for (int i = 0; i < nc; i++)
@@ -1692,6 +1738,17 @@ int usg_forces(usg_ctx *c, double *cl, double *cd) {
   return USG_OK;
 }
 
+int usg_case_status(usg_ctx *c, int *status) {
+  if (!c || !status) return fail(USG_ERR_ARG, "null argument");
+  if (c->ncase == 1) {
+    status[0] = 0;
+    return USG_OK;
+  }
+  HIPCHK(hipMemcpyAsync(status, c->ds.err + 2, c->ncase * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return USG_OK;
+}
+
 int usg_case_count(usg_ctx *c, int *ncases, int *cell_off) {
   if (!c || !ncases) return fail(USG_ERR_ARG, "null argument");
   *ncases = c->ncase;
@@ -1747,6 +1804,102 @@ int usg_use_graphs(usg_ctx *c, int on) {
     drop_graphs(c);
     return fail(USG_ERR_HIP, "hipGraph capture of the LU-SGS iteration pair failed");
   }
+  return USG_OK;
+}
+
+int usg_set_block(usg_ctx *c, int group, int threads) {
+  if (!c || group < 0 || group > USG_K_SWEEP) return fail(USG_ERR_ARG, "bad kernel group");
+  if (threads != 64 && threads != 128 && threads != 256)
+    return fail(USG_ERR_ARG, "threads per block must be 64, 128 or 256");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->blk[group] = threads;
+  drop_graphs(c);
+  return USG_OK;
+}
+
+int usg_get_block(usg_ctx *c, int group, int *threads) {
+  if (!c || !threads || group < 0 || group > USG_K_SWEEP) return fail(USG_ERR_ARG, "bad argument");
+  *threads = c->blk[group];
+  return USG_OK;
+}
+
+// Mean milliseconds of one eager LU-SGS iteration with the current blocks.
+static int time_iterations(usg_ctx *c, int iters, hipEvent_t a, hipEvent_t b, float *ms) {
+  one_iteration(c, nullptr, nullptr);  // warm-up (code objects, caches)
+  HIPCHK(hipEventRecord(a, c->stream));
+  for (int i = 0; i < iters; i++) one_iteration(c, nullptr, nullptr);
+  HIPCHK(hipEventRecord(b, c->stream));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  *ms /= iters;
+  return check_err(c);
+}
+
+int usg_autotune(usg_ctx *c, int iters, int *best) {
+  if (!c || iters <= 0) return fail(USG_ERR_ARG, "bad argument");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int nc = c->hm.nc;
+  const bool sst = c->prm.flow == USG_SST;
+  // the iterations below advance the solution: keep the state to restore it
+  std::vector<double *> live = {c->ds.U, c->ds.W};
+  std::vector<size_t> len = {4 * (size_t)nc, 4 * (size_t)nc};
+  if (sst) {
+    live.insert(live.end(), {c->ds.tU, c->ds.tW, c->ds.mut});
+    len.insert(len.end(), {2 * (size_t)nc, 2 * (size_t)nc, (size_t)nc});
+  }
+  double *const startU = c->ds.U;
+  std::vector<std::vector<double>> keep(live.size());
+  for (size_t i = 0; i < live.size(); i++) {
+    keep[i].resize(len[i]);
+    HIPCHK(hipMemcpy(keep[i].data(), live[i], len[i] * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    hipEventDestroy(a);
+    return fail(USG_ERR_HIP, "hipEventCreate");
+  }
+  drop_graphs(c);
+  int rc = USG_OK;
+  const int cand[3] = {64, 128, 256};
+  for (int g = 0; g <= USG_K_SWEEP && rc == USG_OK; g++) {
+    int pick_t = c->blk[g];
+    float pick_ms = 1e30f;
+    for (int t : cand) {
+      c->blk[g] = t;
+      float ms = 0;
+      rc = time_iterations(c, iters, a, b, &ms);
+      if (rc) break;
+      if (ms < pick_ms) {
+        pick_ms = ms;
+        pick_t = t;
+      }
+    }
+    c->blk[g] = pick_t;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  // back to the starting buffers and state
+  if (c->ds.U != startU) swap_state(c);
+  for (size_t i = 0; i < live.size(); i++) {
+    double *dst = i == 0 ? c->ds.U : i == 1 ? c->ds.W : i == 2 ? c->ds.tU : i == 3 ? c->ds.tW : c->ds.mut;
+    HIPCHK(hipMemcpy(dst, keep[i].data(), len[i] * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipDeviceSynchronize());
+  c->have_residual = false;
+  if (rc) return rc;
+  if (best)
+    for (int g = 0; g <= USG_K_SWEEP; g++) best[g] = c->blk[g];
+  return USG_OK;
+}
+
+int usg_device_arch(char *buf, int n) {
+  if (!buf || n <= 0) return fail(USG_ERR_ARG, "bad argument");
+  int dev = 0;
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  std::snprintf(buf, n, "%s", p.gcnArchName);
   return USG_OK;
 }
 

@@ -415,7 +415,7 @@ def naca4(t: float, m_camber: float = 0.0, p: float = 0.4, n: int = 256) -> np.n
 
 def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: float = 4.5e-6,
                    rfar: float = 50.0, growth: float = 1.15, seed: int | None = None,
-                   jitter: float = 0.0, quad_only: bool = False) -> Mesh:
+                   jitter: float = 0.0, quad_only: bool = False, blend_r: float = 1.0) -> Mesh:
     """Hybrid O-mesh around a 4-digit airfoil (chord 1, far field at ``rfar``).
 
     Layers leave the wall with first spacing ``dy1`` growing by ``growth``
@@ -424,7 +424,11 @@ def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: flo
     that spacing, so the outer cells have aspect ratio ~1 out to the far field.
     The anisotropic wall layers are quadrilaterals, the isotropic region is
     split into Delaunay triangles (kept as quadrilaterals with
-    ``quad_only``).  ``ni`` points around the contour.
+    ``quad_only``).  ``ni`` points around the contour.  Layers are marched
+    out to ``blend_r`` chords; beyond, each point is blended linearly from
+    its position on that (nearly convex) layer to the far-field circle at the
+    same angle, which cannot fold the way long marches from a concave
+    (cambered) surface do.
     """
     c = naca4(t, camber, n=ni)
     c = c[:-1] if np.allclose(c[0], c[-1]) else c
@@ -485,6 +489,22 @@ def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: flo
                 tng /= np.linalg.norm(tng, axis=1)[:, None]
                 cur = cur + mu * (lap * tng).sum(1)[:, None] * tng
         pts[:, j] = cur
+        if eta[j] >= blend_r and j < nj:
+            jb = j
+            break
+    else:
+        jb = nj
+    if jb < nj:
+        base = pts[:, jb]
+        cm = base.mean(axis=0)
+        th = np.unwrap(np.arctan2(base[:, 1] - cm[1], base[:, 0] - cm[0]))
+        if not (np.all(np.diff(th) > 0) or np.all(np.diff(th) < 0)):
+            raise ValueError("airfoil_hybrid: blend layer is not star-shaped; raise blend_r")
+        r_out = rfar + 0.5
+        outer = cm + r_out * np.column_stack([np.cos(th), np.sin(th)])
+        for j in range(jb + 1, nj + 1):
+            w = (eta[j] - eta[jb]) / (eta[nj] - eta[jb])
+            pts[:, j] = (1 - w) * base + w * outer
     if seed is not None and jitter > 0:
         rng = np.random.default_rng(seed)
         dr = np.gradient(eta)[None, :, None]
@@ -522,6 +542,26 @@ def airfoil_1m() -> Mesh:
     (y+ ~ 1 at Re 6e6), quad wall layers, isotropic triangles out to a far
     field at 50 chords, 810 points around the contour."""
     return airfoil_hybrid(ni=810, t=0.12)
+
+
+def batch_cases(n: int = 64, seed: int = 4004, nc_min: float = 2e4, nc_max: float = 4e5) -> list[dict]:
+    """Parameters of a batch of airfoil cases: cell counts log-uniform in
+    [nc_min, nc_max] (through the contour resolution ni, ~1.52 ni^2 cells for
+    the hybrid y+~1 mesh), 4-digit thickness U(0.06, 0.18), camber U(0, 0.04)
+    at p = 0.4, Mach U(0.3, 0.8), angle of attack U(-4, 8) degrees."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        nc = float(np.exp(rng.uniform(np.log(nc_min), np.log(nc_max))))
+        out.append({"ni": max(32, int(round(np.sqrt(nc / 1.52)))), "t": float(rng.uniform(0.06, 0.18)),
+                    "camber": float(rng.uniform(0.0, 0.04)), "mach": float(rng.uniform(0.3, 0.8)),
+                    "aoa": float(rng.uniform(-4.0, 8.0))})
+    return out
+
+
+def batch_mesh(case: dict) -> Mesh:
+    """The hybrid y+~1 mesh of one batch case (first spacing 4.5e-6 c, far field 50 c)."""
+    return airfoil_hybrid(ni=case["ni"], t=case["t"], camber=case["camber"])
 
 
 def rect(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0, bottom: int = FARFIELD, top: int = FARFIELD,

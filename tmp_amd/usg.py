@@ -26,6 +26,7 @@ EULER, LAMINAR, SST = 0, 1, 2
 FP64, MIXED = 0, 1
 F_PRIM, F_CONS, F_RES, F_GRAD, F_LAMC, F_PHI, F_DW1, F_DW = range(8)
 F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW = range(8, 16)
+K_GRAD, K_FACE, K_CELL, K_SWEEP = range(4)
 
 _FIELD_WIDTH = {F_PRIM: 4, F_CONS: 4, F_RES: 4, F_GRAD: 8, F_LAMC: 1, F_PHI: 4, F_DW1: 4, F_DW: 4,
                 F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2}
@@ -52,7 +53,8 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_set_prim", "usg_residual", "usg_lusgs_step", "usg_rk5_step", "usg_get_field", "usg_res_l2",
            "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
-           "usg_batch_create", "usg_case_count"]
+           "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
+           "usg_device_arch", "usg_case_status"]
 
 _lib = None
 
@@ -87,6 +89,11 @@ def lib():
         L.usg_use_graphs.argtypes = [vp, C.c_int]
         L.usg_batch_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_Params), C.c_int, C.POINTER(vp)]
         L.usg_case_count.argtypes = [vp, _ip, _ip]
+        L.usg_set_block.argtypes = [vp, C.c_int, C.c_int]
+        L.usg_get_block.argtypes = [vp, C.c_int, _ip]
+        L.usg_autotune.argtypes = [vp, C.c_int, _ip]
+        L.usg_device_arch.argtypes = [C.c_char_p, C.c_int]
+        L.usg_case_status.argtypes = [vp, _ip]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -226,6 +233,24 @@ class Solver:
         _check(lib().usg_kernel_time(self._h, C.byref(t), C.byref(n)))
         return t.value, n.value
 
+    # --- launch configuration -------------------------------------------
+    def blocks(self) -> list:
+        out = []
+        for g in range(4):
+            v = C.c_int()
+            _check(lib().usg_get_block(self._h, g, C.byref(v)))
+            out.append(v.value)
+        return out
+
+    def set_blocks(self, blocks) -> None:
+        for g, t in enumerate(blocks):
+            _check(lib().usg_set_block(self._h, g, int(t)))
+
+    def autotune(self, iters: int = 5) -> list:
+        best = np.zeros(4, np.int32)
+        _check(lib().usg_autotune(self._h, iters, best.ctypes.data_as(_ip)))
+        return best.tolist()
+
 
 class BatchSolver(Solver):
     """Many independent cases in one context (usg_batch_create): per-case
@@ -255,6 +280,12 @@ class BatchSolver(Solver):
         self.ncase = n.value
         self.case_cell = off.astype(np.int64)
         self.nc = int(off[-1])
+
+    def status(self) -> np.ndarray:
+        """Per-case status: 0 running, bit 0 HLLC p* check failed, bit 1 slip-wall check failed."""
+        st = np.zeros(self.ncase, np.int32)
+        _check(lib().usg_case_status(self._h, st.ctypes.data_as(_ip)))
+        return st
 
     def case_slice(self, i: int) -> slice:
         return slice(int(self.case_cell[i]), int(self.case_cell[i + 1]))
