@@ -12,13 +12,18 @@ from tmp_amd import usg
 
 from .common import part2
 
-HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "usg.h")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
 
 
-def declared():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^[a-z][\w\s\*]*?\b(usg_\w+)\s*\(", text, flags=re.M)))
+def declared(name=None):
+    """Functions declared by include/<name> (every header when None)."""
+    names = set()
+    for h in sorted(os.listdir(INCLUDE)):
+        if not h.endswith(".h") or (name and h != name):
+            continue
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(INCLUDE, h)).read(), flags=re.S)
+        names |= set(re.findall(r"^[a-z][\w\s\*]*?\b(usg_\w+)\s*\(", text, flags=re.M))
+    return sorted(names)
 
 
 def test_header_parses():
@@ -33,8 +38,10 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
-def test_python_binds_exactly_the_header():
-    assert sorted(usg.EXPORTS) == declared()
+def test_python_binds_exactly_the_headers():
+    from tmp_amd import mlp
+    assert sorted(usg.EXPORTS) == declared("usg.h")
+    assert sorted(mlp.EXPORTS) == declared("usg_mlp.h")
 
 
 def _params(flow=usg.EULER, precision=usg.FP64):
