@@ -83,9 +83,11 @@ def _cpu_case(args):
 def cpu_baseline_batch(cases, flow, re, seconds, workers):
     """One batch case per core (the oracle, 1 thread each) on ``workers``
     cores at once, each for ``seconds``: summed cell-iterations/s."""
+    import multiprocessing
     from concurrent.futures import ProcessPoolExecutor
     sample = cases[:workers]
-    with ProcessPoolExecutor(max_workers=workers) as ex:
+    # spawned (not forked) workers: this process already holds the GPU
+    with ProcessPoolExecutor(max_workers=workers, mp_context=multiprocessing.get_context("spawn")) as ex:
         res = list(ex.map(_cpu_case, [(c, flow, re, seconds) for c in sample]))
     return sum(r[0] for r in res), sum(r[1] for r in res), len(sample)
 
