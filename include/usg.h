@@ -180,12 +180,27 @@ int usg_device_arch(char *buf, int n);
 int usg_timing_enable(usg_ctx *ctx, int on);
 int usg_kernel_time(usg_ctx *ctx, double *total_ms, long *launches);
 
-/* Overset donor-stencil apply (UniTioga/dataUpdate.cpp:6-133,
- * MeshBlock::getInterpolatedSolution + updateSolnData, ROW layout):
- * receptor r takes q[recv[r]*nvar + k] = sum_{m<nw[r]} w[r*4+m] * q[donor[r*4+m]*nvar + k].
- * Works on a caller-provided device array q of nq*nvar doubles. */
-int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const int *nw,
-                      const int *donor, const double *w, void *hip_stream);
+/* Overset donor-stencil apply (MeshBlock::getInterpolatedSolution +
+ * updateSolnData, UniTioga/dataUpdate.cpp:6-133) on device arrays:
+ *   dst[recv[r]][k] = sum_{m < nw[r]} w[r*4+m] * src[donor[r*4+m]][k]
+ * for r < nrecv, k < nvar, summed in m order from 0.  layout USG_ROW stores
+ * point i's variables at q[i*nvar + k] (the reference's ROW), USG_COLUMN at
+ * q[k*npts + i] (the solver's own layout); src_pts / dst_pts are the arrays'
+ * point counts.  recv, nw, donor, w are device arrays.  Entries whose
+ * indices fall outside the arrays are skipped.  If src and dst are the same
+ * array, no receptor may also be a donor (receptors are fringe cells,
+ * donors field cells, as in an overset assembly). */
+#define USG_ROW 0
+#define USG_COLUMN 1
+int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, long src_pts, long dst_pts,
+                      int nrecv, const int *recv, const int *nw, const int *donor, const double *w,
+                      void *hip_stream);
+/* Donor weights of n receptor points (host arrays): point r lies in a cell
+ * of nvert[r] (3 or 4) vertices xv[r*8 .. r*8+2*nvert[r]) and gets
+ * w[r*4 .. r*4+nvert[r]) -- computeNodalWeight2D (mathTools/math.cpp:229-312):
+ * barycentric for a triangle, bilinear (Newton, |F| <= 1e-14, 500 steps) for
+ * a quadrilateral.  Host only, no GPU needed. */
+int usg_nodal_weights(int n, const int *nvert, const double *xv, const double *xp, double *w);
 
 /* Plain device copy kernel (n = bytes/8 doubles) on a stream: the reference
  * kernel of known traffic used to calibrate HBM byte counters. */

@@ -69,6 +69,8 @@ def lib():
         if not os.path.exists(LIB):
             build()
         L = C.CDLL(LIB)
+        L.orc_nodal_weights.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.orc_stencil_apply.argtypes = [_dp, _dp, C.c_int, C.c_int, _ip, _ip, _ip, _dp]
         L.orc_mesh_create.restype = C.POINTER(_Mesh)
         L.orc_mesh_create.argtypes = [C.c_int, _dp, C.c_int, _ip, _ip, _ip, C.c_int]
         L.orc_mesh_destroy.argtypes = [C.POINTER(_Mesh)]
@@ -219,3 +221,24 @@ def cons_to_prim(W):
 def cell_volume(xy):
     xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
     return lib().orc_cell_volume(xy.ctypes.data_as(_dp), xy.shape[0])
+
+
+def nodal_weights(nvert, xv, xp):
+    """orc_nodal_weights of one point: weights[nvert] (or None on a degenerate cell)."""
+    xv = np.ascontiguousarray(xv, np.float64).reshape(-1)
+    xp = np.ascontiguousarray(xp, np.float64).reshape(2)
+    w = np.zeros(4)
+    rc = lib().orc_nodal_weights(int(nvert), xv.ctypes.data_as(_dp), xp.ctypes.data_as(_dp), w.ctypes.data_as(_dp))
+    return None if rc else w[:nvert]
+
+
+def stencil_apply(src, dst, recv, nw, donor, w):
+    """orc_stencil_apply, ROW layout: src[npts, nvar] -> dst[npts', nvar] in place."""
+    src = np.ascontiguousarray(src, np.float64)
+    nvar = src.shape[1]
+    recv, nw, donor = (np.ascontiguousarray(a, np.int32) for a in (recv, nw, donor))
+    w = np.ascontiguousarray(w, np.float64)
+    lib().orc_stencil_apply(src.ctypes.data_as(_dp), dst.ctypes.data_as(_dp), nvar, recv.size,
+                            recv.ctypes.data_as(_ip), nw.ctypes.data_as(_ip), donor.ctypes.data_as(_ip),
+                            w.ctypes.data_as(_dp))
+    return dst

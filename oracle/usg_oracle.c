@@ -1426,3 +1426,123 @@ void orc_lusgs_levels(const orc_mesh *m, int *flevel, int *blevel, int *nfl, int
   }
   *nbl = mx;
 }
+
+/* ------------------------------------------------------------------------
+ * Overset stencil: computeNodalWeight2D (mathTools/math.cpp:229-312) with
+ * solvec (:4-58), newtonSolve2D (:126-188) and findClockWise (:189-227), and
+ * MeshBlock::getInterpolatedSolution + updateSolnData
+ * (UniTioga/dataUpdate.cpp:6-133): q_r[k] = sum_m w_m q[donor_m][k].
+ * ------------------------------------------------------------------------ */
+static int orc_solvec2(double a[2][2], double b[2]) {
+  int i, j, k, l;
+  for (i = 0; i < 2; i++) {
+    if (fabs(a[i][i]) < 1e-8) {
+      int flag = 1;
+      for (k = i + 1; k < 2 && flag; k++) {
+        if (!(fabs(a[k][i]) < 1e-8)) {
+          double t;
+          flag = 0;
+          for (l = 0; l < 2; l++) {
+            t = a[k][l];
+            a[k][l] = a[i][l];
+            a[i][l] = t;
+          }
+          t = b[k];
+          b[k] = b[i];
+          b[i] = t;
+        }
+      }
+      if (flag) return 0;
+    }
+    for (k = i + 1; k < 2; k++) {
+      double fact = -a[k][i] / a[i][i];
+      for (j = 0; j < 2; j++) a[k][j] += fact * a[i][j];
+      b[k] += fact * b[i];
+    }
+  }
+  for (i = 1; i >= 0; i--) {
+    double sum = 0;
+    for (j = i + 1; j < 2; j++) sum += a[i][j] * b[j];
+    b[i] = (b[i] - sum) / a[i][i];
+  }
+  return 1;
+}
+
+static double orc_crossz(const double *o, const double *p, const double *q) {
+  return (p[0] - o[0]) * (q[1] - o[1]) - (p[1] - o[1]) * (q[0] - o[0]);
+}
+
+int orc_nodal_weights(int nvert, const double *xv, const double *xp, double *frac) {
+  int i;
+  if (nvert == 3) {
+    double lhs[2][2], rhs[2];
+    int j;
+    for (i = 0; i < 2; i++) {
+      for (j = 0; j < 2; j++) lhs[i][j] = xv[2 * j + i] - xv[4 + i];
+      rhs[i] = xp[i] - xv[4 + i];
+    }
+    if (!orc_solvec2(lhs, rhs)) return -4;
+    frac[0] = rhs[0];
+    frac[1] = rhs[1];
+    frac[2] = 1. - frac[0] - frac[1];
+    return 0;
+  }
+  if (nvert == 4) {
+    int map[4] = {0, 1, 2, 3}, iter, ok = 1;
+    double f[4][2], u = 0.5, v = 0.5;
+    double ab_ac = orc_crossz(xv, xv + 2, xv + 4), ab_ad = orc_crossz(xv, xv + 2, xv + 6);
+    if (ab_ac * ab_ad > 0) {
+      if (!(orc_crossz(xv, xv + 4, xv + 6) * ab_ad > 0)) {
+        map[2] = 3;
+        map[3] = 2;
+      }
+    } else {
+      map[1] = 2;
+      map[2] = 1;
+    }
+    for (i = 0; i < 2; i++) {
+      f[0][i] = xv[2 * map[0] + i] - xp[i];
+      f[1][i] = xv[2 * map[1] + i] - xv[2 * map[0] + i];
+      f[2][i] = xv[2 * map[3] + i] - xv[2 * map[0] + i];
+      f[3][i] = xv[2 * map[0] + i] - xv[2 * map[1] + i] + xv[2 * map[2] + i] - xv[2 * map[3] + i];
+    }
+    for (iter = 0; iter < 500; iter++) {
+      double uv = u * v, rhs[2], lhs[2][2], norm;
+      for (i = 0; i < 2; i++) rhs[i] = f[0][i] + f[1][i] * u + f[2][i] * v + f[3][i] * uv;
+      norm = rhs[0] * rhs[0] + rhs[1] * rhs[1];
+      if (sqrt(norm) <= 1e-14) break;
+      for (i = 0; i < 2; i++) {
+        lhs[i][0] = f[1][i] + f[3][i] * v;
+        lhs[i][1] = f[2][i] + f[3][i] * u;
+      }
+      if (!orc_solvec2(lhs, rhs)) {
+        ok = 0;
+        break;
+      }
+      u -= rhs[0] * 1.0;
+      v -= rhs[1] * 1.0;
+    }
+    if (iter == 500 || !ok) {
+      u = 2.0;
+      v = 0.0;
+    }
+    frac[map[0]] = (1 - u) * (1 - v);
+    frac[map[1]] = u * (1 - v);
+    frac[map[2]] = u * v;
+    frac[map[3]] = (1 - u) * v;
+    return 0;
+  }
+  return -1;
+}
+
+/* ROW layout q[i*nvar + k] (getInterpolatedSolution's interptype ROW) */
+void orc_stencil_apply(const double *src, double *dst, int nvar, int nrecv, const int *recv, const int *nw,
+                       const int *donor, const double *w) {
+  int r, k, m;
+  for (r = 0; r < nrecv; r++)
+    for (k = 0; k < nvar; k++) {
+      double qq = 0;
+      for (m = 0; m < nw[r]; m++) qq += src[(long)donor[4 * r + m] * nvar + k] * w[4 * r + m];
+      dst[(long)recv[r] * nvar + k] = qq;
+    }
+}

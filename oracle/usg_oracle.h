@@ -96,6 +96,12 @@ void orc_hllc_face(const double *UL, const double *UR, const double *n, double a
                    double *flux, int *pstar_ok);
 void orc_lusgs_levels(const orc_mesh *m, int *flevel, int *blevel, int *nfl, int *nbl);
 
+/* overset stencil: donor weights of a point in a triangle / quadrilateral
+ * (xv[nvert*2]) and the interpolation q_dst[recv] = sum w q_src[donor] */
+int orc_nodal_weights(int nvert, const double *xv, const double *xp, double *frac);
+void orc_stencil_apply(const double *src, double *dst, int nvar, int nrecv, const int *recv, const int *nw,
+                       const int *donor, const double *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -919,15 +919,27 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevSta
     out2[2 * blockIdx.x + 1] = sy[0];
   }
 }
-__global__ __launch_bounds__(kBlock) void k_stencil(double *q, int nvar, int nrecv, const int *recv, const int *nw,
+// Donor-stencil apply (MeshBlock::getInterpolatedSolution + updateSolnData,
+// UniTioga/dataUpdate.cpp:6-133): dst[recv[r]][k] = sum_m w[r][m] src[donor[r][m]][k],
+// weights in order from 0; ROW (q[i*nvar + k]) or COLUMN (q[k*npts + i],
+// the solver's own layout).  Entries with indices outside the arrays are
+// skipped, so a bad stencil cannot fault.
+__global__ __launch_bounds__(kBlock) void k_stencil(const double *src, double *dst, int nvar, int column, long sp,
+                                                    long dp, int nrecv, const int *recv, const int *nw,
                                                     const int *donor, const double *w) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nrecv * nvar) return;
-  const int r = t / nvar, kv = t - r * nvar;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)nrecv * nvar) return;
+  const int r = (int)(t / nvar), kv = (int)(t - (long)r * nvar);
+  const int rc = recv[r];
+  if (rc < 0 || rc >= dp) return;
+  const int n = min(max(nw[r], 0), 4);
   double acc = 0;
-  const int n = nw[r];
-  for (int mm = 0; mm < n; mm++) acc += q[(long)donor[4 * r + mm] * nvar + kv] * w[4 * r + mm];
-  q[(long)recv[r] * nvar + kv] = acc;
+  for (int mm = 0; mm < n; mm++) {
+    const int d = donor[4 * r + mm];
+    if (d < 0 || d >= sp) continue;
+    acc += src[column ? (long)kv * sp + d : (long)d * nvar + kv] * w[4 * r + mm];
+  }
+  dst[column ? (long)kv * dp + rc : (long)rc * nvar + kv] = acc;
 }
 
 // Plain copy of n doubles: the reference kernel for calibrating traffic counters.
@@ -1925,14 +1937,17 @@ int usg_kernel_time(usg_ctx *c, double *total_ms, long *launches) {
   return USG_OK;
 }
 
-int usg_stencil_apply(void *dev_q, int nvar, int nrecv, const int *recv, const int *nw, const int *donor,
-                      const double *w, void *st) {
-  if (!dev_q || nvar <= 0 || nrecv < 0 || (nrecv && (!recv || !nw || !donor || !w)))
+int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, long src_pts, long dst_pts,
+                      int nrecv, const int *recv, const int *nw, const int *donor, const double *w, void *st) {
+  if (!dev_src || !dev_dst || nvar <= 0 || nrecv < 0 || src_pts < 0 || dst_pts < 0 ||
+      (layout != USG_ROW && layout != USG_COLUMN) || (nrecv && (!recv || !nw || !donor || !w)))
     return fail(USG_ERR_ARG, "bad argument");
+  if (src_pts > 2147483647L || dst_pts > 2147483647L) return fail(USG_ERR_ARG, "point count exceeds int32");
   if (nrecv == 0) return USG_OK;
-  hipStream_t stream = (hipStream_t)st;
-  hipLaunchKernelGGL(k_stencil, dim3(grid(nrecv * nvar)), dim3(kBlock), 0, stream, (double *)dev_q, nvar, nrecv,
-                     recv, nw, donor, w);
+  const long nt = (long)nrecv * nvar;
+  hipLaunchKernelGGL(k_stencil, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, (hipStream_t)st,
+                     (const double *)dev_src, (double *)dev_dst, nvar, layout == USG_COLUMN ? 1 : 0, src_pts, dst_pts,
+                     nrecv, recv, nw, donor, w);
   HIPCHK(hipGetLastError());
   return USG_OK;
 }

@@ -54,7 +54,7 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
-           "usg_device_arch", "usg_case_status"]
+           "usg_device_arch", "usg_case_status", "usg_nodal_weights"]
 
 _lib = None
 
@@ -84,7 +84,8 @@ def lib():
         L.usg_mesh_build.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, _dp, _dp, _dp, _dp, _ip, _ip]
         L.usg_timing_enable.argtypes = [vp, C.c_int]
         L.usg_kernel_time.argtypes = [vp, _dp, C.POINTER(C.c_long)]
-        L.usg_stencil_apply.argtypes = [vp, C.c_int, C.c_int, _ip, _ip, _ip, _dp, vp]
+        L.usg_stencil_apply.argtypes = [vp, vp, C.c_int, C.c_int, C.c_long, C.c_long, C.c_int, vp, vp, vp, vp, vp]
+        L.usg_nodal_weights.argtypes = [C.c_int, _ip, _dp, _dp, _dp]
         L.usg_copy_kernel.argtypes = [vp, vp, C.c_long, vp]
         L.usg_use_graphs.argtypes = [vp, C.c_int]
         L.usg_batch_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_Params), C.c_int, C.POINTER(vp)]
@@ -122,6 +123,31 @@ def mesh_geometry(m: _mesh.Mesh) -> dict:
                                               ("vol", "cc", "fn", "fa", "fc", "wdist")), C.byref(a), C.byref(b)))
     out["nfwd"], out["nbwd"] = a.value, b.value
     return out
+
+
+ROW, COLUMN = 0, 1
+
+
+def nodal_weights(nvert, xv, xp) -> np.ndarray:
+    """Donor weights w[n, 4] of points xp[n, 2] in cells xv[n, 4, 2] of
+    nvert[n] (3 or 4) vertices (computeNodalWeight2D)."""
+    nvert = np.ascontiguousarray(nvert, np.int32)
+    n = nvert.size
+    X = np.zeros((n, 4, 2))
+    X[:, :np.asarray(xv).shape[1]] = xv
+    X = np.ascontiguousarray(X)
+    P = np.ascontiguousarray(xp, np.float64).reshape(n, 2)
+    w = np.zeros((n, 4))
+    _check(lib().usg_nodal_weights(n, nvert.ctypes.data_as(_ip), X.ctypes.data_as(_dp), P.ctypes.data_as(_dp),
+                                   w.ctypes.data_as(_dp)))
+    return w
+
+
+def stencil_apply(src_ptr, dst_ptr, nvar, layout, src_pts, dst_pts, nrecv, recv_ptr, nw_ptr, donor_ptr, w_ptr,
+                  stream=None):
+    """usg_stencil_apply on device pointers (ints)."""
+    _check(lib().usg_stencil_apply(src_ptr, dst_ptr, nvar, layout, src_pts, dst_pts, nrecv, recv_ptr, nw_ptr,
+                                   donor_ptr, w_ptr, stream))
 
 
 class Solver:
