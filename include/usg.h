@@ -135,8 +135,12 @@ int usg_rk5_step(usg_ctx *ctx, int n, double *res_hist);
 int usg_get_field(usg_ctx *ctx, int field, double *out, long count);
 /* ||R_rho / V||_2 of the last residual (the convergence norm), out[ncases]. */
 int usg_res_l2(usg_ctx *ctx, double *out);
-/* Lift and drag coefficients of the current state from the wall faces'
- * pressure force, cl[ncases], cd[ncases]. */
+/* Lift and drag coefficients of the current state, cl[ncases], cd[ncases]:
+ * the momentum flux out of the fluid through the wall faces -- p n A with
+ * the wall cell's pressure, minus the viscous flux tau . n A for viscous
+ * flow (which evaluates the residual of the current state first) --
+ * rotated by the angle of attack and scaled by 1/2 gamma M^2.  The
+ * reference integrates no forces; this is the definition used here. */
 int usg_forces(usg_ctx *ctx, double *cl, double *cd);
 
 /* Mesh facts: number of LU-SGS waves (forward, backward) of the numbering. */

@@ -50,7 +50,7 @@ class _Solver(C.Structure):
                                       "mut_face", "tFc", "tFv", "tR", "tD", "tW", "tdW1", "tdW"]]
                 + [(n, C.c_double) for n in ["fs_mut", "fsnd_mut", "fsnd_mu"]]
                 + [("fs_turb", C.c_double * 2), ("fsnd_turb", C.c_double * 2)]
-                + [("hllc_pstar_violations", C.c_long)])
+                + [("hllc_pstar_violations", C.c_long), ("Fv_face", _dp)])
 
 
 def build(force: bool = False) -> str:

@@ -320,6 +320,7 @@ orc_solver *orc_solver_create(orc_mesh *m, int flow, double ma, double aoa_deg, 
   s->F1 = A(nc); s->F2 = A(nc); s->mut = A(nc); s->mut_face = A(nf);
   s->tFc = A(nf * 2); s->tFv = A(nf * 2); s->tR = A(nc * 2);
   s->tD = A(nc * 2); s->tW = A(nc * 2); s->tdW1 = A(nc * 2); s->tdW = A(nc * 2);
+  s->Fv_face = A(nf * 2);
 #undef A
   /* LUSGS_Integrator.h:18 ; RungeKuttaStrategy.h:122-131 */
   s->cfl_lusgs = 400;
@@ -338,7 +339,7 @@ void orc_solver_destroy(orc_solver *s) {
                     s->Umax, s->Umin, s->phi, s->UL, s->UR, s->mu, s->mu_edge, s->D,
                     s->W, s->dW1, s->dW, s->W0, s->Wnow, s->tU, s->tUedge, s->tgrad,
                     s->F1, s->F2, s->mut, s->mut_face, s->tFc, s->tFv, s->tR, s->tD,
-                    s->tW, s->tdW1, s->tdW};
+                    s->tW, s->tdW1, s->tdW, s->Fv_face};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
   free(s);
 }
@@ -842,6 +843,8 @@ static void laminar_flux(orc_solver *s) {
     Fv[1] = ((ve0 * taoxx + ve1 * taoxy - qx) * n0 + (ve0 * taoxy + ve1 * taoyy - qy) * n1) * A;
     Fv[2] = (taoxx * n0 + taoxy * n1) * A;
     Fv[3] = (taoxy * n0 + taoyy * n1) * A;
+    s->Fv_face[2 * k] = Fv[2];
+    s->Fv_face[2 * k + 1] = Fv[3];
     for (int j = 0; j < NE; j++) s->R[j + NE * lC] -= Fv[j];
     if (rC >= 0)
       for (int j = 0; j < NE; j++) s->R[j + NE * rC] += Fv[j];
@@ -1376,20 +1379,24 @@ double orc_dw_rho_sq(orc_solver *s) {
   return acc;
 }
 
-/* Integrated wall force (the build's definition; the reference has none):
- * momentum entries of the wall faces' residual contributions, p n A for the
- * inviscid wall flux (HLLCFluxStrategy.cpp:126-135) minus the viscous face
- * flux; C_L, C_D rotated by the angle of attack and scaled by 1/2 gamma M^2. */
+/* Integrated wall force (the reference has none): the momentum flux out of
+ * the fluid through the wall faces of the last residual -- the inviscid wall
+ * flux p n A (HLLCFluxStrategy.cpp:126-135) minus, for viscous flow, the
+ * viscous face flux tau . n A (LaminarFluxStrategy.h:182-186); C_L, C_D
+ * rotated by the angle of attack and scaled by 1/2 gamma M^2. */
 void orc_forces(orc_solver *s, double *cl, double *cd) {
   const orc_mesh *m = s->m;
   double fx = 0, fy = 0;
   for (int k = 0; k < m->nf; k++) {
     if (m->frc[k] != ORC_WALL) continue;
-    int lC = m->flc[k];
     double p = s->U_edge[1 + NE * k];
-    (void)lC;
-    fx += p * m->fn[2 * k] * m->fa[k];
-    fy += p * m->fn[2 * k + 1] * m->fa[k];
+    double gx = p * m->fn[2 * k] * m->fa[k], gy = p * m->fn[2 * k + 1] * m->fa[k];
+    if (!is_inviscid(s)) {
+      gx = gx - s->Fv_face[2 * k];
+      gy = gy - s->Fv_face[2 * k + 1];
+    }
+    fx += gx;
+    fy += gy;
   }
   double q = 0.5 * s->Gamma * s->fs_mach * s->fs_mach;
   double ca = cos(s->fs_AOA), sa = sin(s->fs_AOA);

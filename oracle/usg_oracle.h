@@ -69,6 +69,8 @@ typedef struct orc_solver {
   double fs_mut, fsnd_mut, fsnd_mu, fs_turb[2], fsnd_turb[2];
   /* invariant counters */
   long hllc_pstar_violations;
+  /* viscous momentum flux of each face (tau . n A), for the wall force */
+  double *Fv_face;
 } orc_solver;
 
 /* mesh / geometry (edge3d_int.h, cell3d_int.h, cellVolume.cpp, CellBrick.cpp) */

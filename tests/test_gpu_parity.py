@@ -121,9 +121,14 @@ def test_deterministic():
     assert np.array_equal(ha, hb)
 
 
-def test_forces_match_oracle():
-    m = part2()
-    orc, gpu, om = _pair(m, O.EULER, 0.5, 2.0, 1e6, perturb=False)
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", [("part2-euler", part2, O.EULER, 0.5, 2.0, 1e6),
+                                                     ("airfoil-laminar", airfoil_small, O.LAMINAR, 0.5, 2.0, 5000.0)],
+                         ids=["part2-euler", "airfoil-laminar"])
+def test_forces_match_oracle(name, mk, flow, mach, aoa, re):
+    """Wall force: pressure p n A, minus the viscous flux tau . n A for
+    laminar flow (tree-summed on the GPU, sequential in the oracle)."""
+    m = mk()
+    orc, gpu, om = _pair(m, flow, mach, aoa, re, perturb=False)
     for _ in range(5):
         orc.lusgs_step()
     gpu.lusgs(5)

@@ -291,3 +291,95 @@ def test_lusgs_matches_dense_factorisation(triangles):
     M = (Dm + Lm) @ np.linalg.inv(Dm) @ (Dm + Um)
     ref = np.linalg.solve(M, -Rt.ravel()).reshape(nc, 4)
     assert np.allclose(dW, ref, rtol=2e-5, atol=1e-6 * np.abs(ref).max())
+
+
+def test_rk5_is_the_closed_form_polynomial_near_free_stream():
+    """RK5 (RungeKuttaStrategy.h:39-72) near the free stream, where R(W_inf)
+    vanishes, is linear: one step maps W_inf + eps w to W_inf + eps P(A) w,
+    A = -(dt/V) dR/dW, with the closed-form amplification polynomial of
+    alpha = (1/4, 1/6, 3/8, 1/2, 1):
+        P(z) = 1 + z + z^2/2 + 3 z^3/16 + z^4/32 + z^5/128.
+    A is applied here through finite differences of the oracle's residual,
+    independently of its stage loop."""
+    m = mesh.prepare(mesh.rect(12, 10, lx=2.0, ly=1.5, jitter=0.15, seed=3, triangles=True))[0]
+    om = O.OracleMesh(m)
+    s = O.OracleSolver(om, O.EULER, 0.5, 3.0, 1e6)
+    Uinf = np.array(list(s.s.fs_primVar)[:4])
+    Winf = O.prim_to_cons(Uinf)
+    rng = np.random.default_rng(7)
+    cc = om.cc.reshape(-1, 2)
+    dw = np.zeros((m.nc, 4))
+    for j in range(4):
+        a, b, ph = rng.uniform(1, 3), rng.uniform(1, 3), rng.uniform(0, 6)
+        dw[:, j] = np.sin(a * cc[:, 0] + ph) * np.cos(b * cc[:, 1])
+    eps = 1e-7
+    vol = om.vol[:, None]
+
+    def R_at(W):
+        s.set_prim(np.array([O.cons_to_prim(w) for w in W]))
+        s.residual()
+        return s.R.copy()
+
+    R0 = R_at(np.tile(Winf, (m.nc, 1)))
+    assert np.max(np.abs(R0)) < 1e-12
+    s.residual()
+    dt = (0.8 * om.vol / s.field("lamc", 1))[:, None]     # CFL 0.8, frozen for the step
+
+    def A(v):
+        return -(dt / vol) * (R_at(Winf + eps * v) - R0) / eps
+
+    v = dw
+    P = dw.copy()
+    for coef in (1.0, 1 / 2, 3 / 16, 1 / 32, 1 / 128):
+        v = A(v)
+        P = P + coef * v
+    s.set_prim(np.array([O.cons_to_prim(w) for w in Winf + eps * dw]))
+    s.rk5_step()
+    W1 = np.array([O.prim_to_cons(u) for u in s.U])
+    step = (W1 - Winf) / eps
+    err = np.max(np.abs(step - P)) / np.max(np.abs(P))
+    # a wrong alpha moves a coefficient by >= 1/32 of |A^k w| (|A| ~ CFL)
+    assert err < 1e-4, err
+    assert np.max(np.abs(P - dw)) > 0.05 * np.max(np.abs(dw))   # the step is not trivial
+
+
+def test_limiter_epsilon_uses_left_volume_to_the_first_power():
+    """Venkatakrishnan eps^2 = K^3 V^(3/ndim) with integer 3/2 == 1, i.e.
+    125 V, taken from the face's LEFT cell even when limiting its right cell
+    (Vankatakrishnan_Limiter.hpp:119,150).  Three cells in a row
+    (west | centre | east), slip walls above and below; the centre is a
+    density maximum, so on its east face Delta1max = 0 and
+    phi = eps^2 / (2 Delta2^2 + eps^2) exactly, with Delta2 = (rho_E - rho_W)/2
+    from the Green-Gauss face averages.  The east face's left cell is the
+    east cell (index 0), whose volume differs from the centre's."""
+    H = 0.01
+    xs = [0.0, 0.015, 0.035, 0.045]          # west [0,.015], centre [.015,.035], east [.035,.045]
+    xy = np.array([[x, 0.0] for x in xs] + [[x, H] for x in xs])
+    # nodes 0-3 bottom, 4-7 top; cells: 0 = east, 1 = centre, 2 = west
+    faces = [  # (p0, p1, left, right): normal (dy, -dx) out of the left cell
+        (6, 2, 0, 1),       # x = .035, out of the east cell (-x)
+        (5, 1, 1, 2),       # x = .015, out of the centre (-x)
+        (2, 3, 0, mesh.WALL), (7, 6, 0, mesh.WALL), (3, 7, 0, mesh.FARFIELD),
+        (1, 2, 1, mesh.WALL), (6, 5, 1, mesh.WALL),
+        (0, 1, 2, mesh.WALL), (5, 4, 2, mesh.WALL), (4, 0, 2, mesh.FARFIELD),
+    ]
+    m = mesh.Mesh(xy=xy, fnode=np.array([f[:2] for f in faces], np.int32),
+                  flc=np.array([f[2] for f in faces], np.int32), frc=np.array([f[3] for f in faces], np.int32),
+                  nc=3, name="row3")
+    om = O.OracleMesh(m)
+    vol = om.vol
+    assert np.allclose(vol, [0.01 * H, 0.02 * H, 0.015 * H], rtol=1e-12)
+    s = O.OracleSolver(om, O.EULER, 0.3, 0.0, 1e6)
+    rho = {0: 0.8, 1: 1.2, 2: 0.5}
+    U = np.array([[rho[c], 1.0, 0.2, 0.0] for c in range(3)])
+    s.set_prim(U)
+    s.residual()
+    phi = s.field("phi", 4)
+    d2 = ((rho[0] + rho[1]) / 2 - (rho[1] + rho[2]) / 2) / 2
+    eps2 = 125.0 * vol[0]                      # east (left) cell, first power
+    expect = eps2 / (2 * d2 * d2 + eps2)
+    assert abs(phi[1, 0] - expect) <= 1e-14
+    # the alternatives differ: the centre's own volume, or V^(3/2)
+    for alt in (125.0 * vol[1], 125.0 * vol[0] ** 1.5):
+        assert abs(alt / (2 * d2 * d2 + alt) - expect) > 0.05
+    assert np.all(phi[1, 1:] == 1.0)           # p, u, v uniform: unlimited

@@ -891,18 +891,27 @@ __global__ void k_set_int(int *p, int v) {
 }
 
 // Wall force: sum over wall faces of (p n A) from the left cell's boundary state.
-// Pressure force p n A over each case's wall faces (one block per case:
-// wall faces [woff[case], woff[case+1])).
+// Wall force over each case's wall faces (one block per case: wall faces
+// [woff[case], woff[case+1])): p n A with the wall cell's pressure, minus
+// the face record's viscous momentum flux for viscous flow.
+template <typename T, bool VISC>
 __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const int *wall_faces,
                                                    const int *wall_cells, const int *woff, double *out2) {
   __shared__ double sx[kBlock], sy[kBlock];
   double fx = 0, fy = 0;
   const int i0 = woff[blockIdx.x], nw = woff[blockIdx.x + 1] - i0;
+  const int nf = m.nf;
+  const T *FF = pickc<T>(s.FF, s.FFf);
   for (int i = threadIdx.x; i < nw; i += kBlock) {
     int f = wall_faces[i0 + i], c = wall_cells[i0 + i];
     double p = s.U[m.nc + c];
-    fx = fx + p * m.fnx[f] * m.fA[f];
-    fy = fy + p * m.fny[f] * m.fA[f];
+    double gx = p * m.fnx[f] * m.fA[f], gy = p * m.fny[f] * m.fA[f];
+    if (VISC) {
+      gx = gx - double(FF[7 * (size_t)nf + f]);
+      gy = gy - double(FF[8 * (size_t)nf + f]);
+    }
+    fx = fx + gx;
+    fy = fy + gy;
   }
   sx[threadIdx.x] = fx;
   sy[threadIdx.x] = fy;
@@ -919,6 +928,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevSta
     out2[2 * blockIdx.x + 1] = sy[0];
   }
 }
+
 // Donor-stencil apply (MeshBlock::getInterpolatedSolution + updateSolnData,
 // UniTioga/dataUpdate.cpp:6-133): dst[recv[r]][k] = sum_m w[r][m] src[donor[r][m]][k],
 // weights in order from 0; ROW (q[i*nvar + k]) or COLUMN (q[k*npts + i],
@@ -1736,8 +1746,20 @@ int usg_res_l2(usg_ctx *c, double *out) {
 
 int usg_forces(usg_ctx *c, double *cl, double *cd) {
   if (!c || !cl || !cd) return fail(USG_ERR_ARG, "null argument");
-  hipLaunchKernelGGL(k_forces, dim3(c->ncase), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->d_wall_faces,
-                     c->d_wall_cells, c->d_woff, c->d_force);
+  const bool visc = c->prm.flow != USG_EULER, mixed = c->prm.precision == USG_MIXED;
+  if (visc) {
+    // the viscous wall flux of the current state: one residual evaluation
+    launch_residual_events(c, nullptr, nullptr);
+    c->have_residual = true;
+  }
+  if (mixed)
+    hipLaunchKernelGGL((visc ? k_forces<float, true> : k_forces<float, false>), dim3(c->ncase), dim3(kBlock), 0,
+                       c->stream, c->dm, c->ds, c->d_wall_faces, c->d_wall_cells, c->d_woff, c->d_force);
+  else
+    hipLaunchKernelGGL((visc ? k_forces<double, true> : k_forces<double, false>), dim3(c->ncase), dim3(kBlock), 0,
+                       c->stream, c->dm, c->ds, c->d_wall_faces, c->d_wall_cells, c->d_woff, c->d_force);
+  int rc = check_err(c);
+  if (rc) return rc;
   std::vector<double> f(2 * (size_t)c->ncase);
   HIPCHK(hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
