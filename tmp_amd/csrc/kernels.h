@@ -48,6 +48,9 @@ struct DevMesh {
   // ccase[c]; both null for a single case (the kernel argument holds them)
   const struct Consts *kc;
   const int *ccase;
+  // bit 0: the cell has no lower neighbour (forward wave 0), bit 1: no
+  // upper neighbour (backward wave 0); those waves are fused into others
+  const unsigned char *wflag;
 };
 
 // State, structure-of-arrays ([var*nc + cell]).

@@ -371,7 +371,8 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
       Fx[j] = T(Fxd[j]);
     }
   }
-  // face record, quantity q at FF[q*nf + f]: flux 0-3, spectral term 4,
+  // face record, quantity q at FF[q*nf + f] (one array per quantity, so a
+  // wave's store of one quantity is contiguous): flux 0-3, spectral term 4,
   // viscous flux 5-8, viscous spectral term 9, SST 10-11
   T *const FF = pick<T>(s.FF, s.FFf) + f;
   for (int j = 0; j < 4; j++) FF[(size_t)j * nf] = Fx[j];
@@ -522,6 +523,94 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
 }
 
 // ---------------------------------------------------------------------------
+// Per-cell ends of the two sweeps, shared by the sweep waves and by the
+// fused first waves (no lower / no upper neighbour: dFi = 0 exactly, so the
+// same expressions give the same bits):
+//   fwd_store: dW1 = (-R - sum_lower DF) / D        (LUSGS_Integrator.cpp:492-523)
+//   bwd_store: dW = dW1 - sum_upper DF / D, W += dW, U = prim(W), SST update
+//              (LUSGS_Integrator.cpp:242-255,526-561, SSTKWStrategy.h:194-223)
+// ---------------------------------------------------------------------------
+template <int FLOW>
+__device__ __forceinline__ void fwd_store(const DevMesh &m, const DevState &s, int c, const double *dFi,
+                                          const double *tFi, double diag, double *dW1, double *tdW1) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  for (int j = 0; j < 4; j++) {
+    dW1[j] = (-s.R[j * nc + c] - dFi[j]) / diag;
+    s.dW1[j * nc + c] = dW1[j];
+  }
+  if (F::sst)
+    for (int q = 0; q < 2; q++) {
+      tdW1[q] = (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
+      s.tdW1[q * nc + c] = tdW1[q];
+    }
+}
+
+template <int FLOW, typename T, bool B>
+__device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, const Consts &k, int c,
+                                          const double *dW1, const double *tdW1, const double *dFi,
+                                          const double *tFi, double diag) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  if constexpr (B) {
+    // a batch case whose residual failed a check in this iteration keeps
+    // its state (the other cases are independent of it)
+    if (s.err[2 + m.ccase[c]]) {
+      for (int j = 0; j < 4; j++) {
+        s.dW[j * nc + c] = 0.0;
+        s.Wn[j * nc + c] = s.W[j * nc + c];
+        s.Un[j * nc + c] = s.U[j * nc + c];
+      }
+      if (F::sst) {
+        for (int q = 0; q < 2; q++) {
+          s.tdW[q * nc + c] = 0.0;
+          s.tWn[q * nc + c] = s.tW[q * nc + c];
+          s.tUn[q * nc + c] = s.tU[q * nc + c];
+        }
+        s.mutn[c] = s.mut[c];
+      }
+      return;
+    }
+  }
+  double dW[4], Wn[4], Un[4];
+  for (int j = 0; j < 4; j++) {
+    dW[j] = dW1[j] - dFi[j] / diag;
+    s.dW[j * nc + c] = dW[j];
+    Wn[j] = s.W[j * nc + c] + dW[j];
+  }
+  prim_of(Wn, Un);
+  store4(s.Wn, nc, c, Wn);
+  store4(s.Un, nc, c, Un);
+  if (F::sst) {
+    double tdw[2], tw[2];
+    for (int q = 0; q < 2; q++) {
+      tdw[q] = tdW1[q] - tFi[q] / s.tD[q * nc + c];
+      s.tdW[q * nc + c] = tdw[q];
+      tw[q] = s.tW[q * nc + c] + tdw[q];
+    }
+    double rho = Un[0];
+    double tu[2] = {tw[0] / rho, tw[1] / rho};
+    // floor at 1e-4 of the free stream (the clip kept commented out at
+    // SSTKWStrategy.h:206-217); a NaN is floored too
+    for (int q = 0; q < 2; q++) {
+      double fl = k.fsnd_turb[q] * 1.0e-4;
+      if (!(tu[q] >= fl)) {
+        tu[q] = fl;
+        tw[q] = rho * fl;
+      }
+    }
+    s.tWn[c] = tw[0];
+    s.tWn[nc + c] = tw[1];
+    s.tUn[c] = tu[0];
+    s.tUn[nc + c] = tu[1];
+    const T *G = pickc<T>(s.grad, s.gradf);
+    double curl = fabs(double(G[6 * nc + c]) - double(G[5 * nc + c]));
+    double h = smax(SST_A1 * tw[1] / rho, double(pickc<T>(s.F2, s.F2f)[c]) * curl);
+    s.mutn[c] = SST_A1 * tw[0] / h;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_cell: residual of one cell as the reference's scatter order gives it --
 // convective face fluxes in ascending face order, then the viscous pass
 // (R_L -= Fv, R_R += Fv, LaminarFluxStrategy.h:192-210), SST source then
@@ -600,44 +689,53 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
     if (F::sst) mprc = mprc + s.mut[c] / k.prt;
     s.mprc[c] = mprc;
   }
-  if (!F::sst) return;
-  double tUc[2] = {s.tU[c], s.tU[nc + c]};
-  const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
-  double tgc[4];
-  for (int q = 0; q < 4; q++) tgc[q] = double(TG[q * nc + c]);
-  const double F1c = double(pickc<T>(s.F1, s.F1f)[c]), mutc = s.mut[c];
-  const double rho = s.U[c];
-  double gux = double(G[4 * nc + c]), guy = double(G[5 * nc + c]);
-  double gvx = double(G[6 * nc + c]), gvy = double(G[7 * nc + c]);
-  double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
-  double ddls1 = (guy + gvx) * (guy + gvx);
-  double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
-  double P = (ddls0 + ddls1) * mutc - ddls2;
-  double rdiv = dot2(tgc[0], tgc[1], tgc[2], tgc[3]);
-  double Cw = F1c * SST_CW1 + (1.0 - F1c) * SST_CW2;
-  double beta = F1c * SST_BETA1 + (1.0 - F1c) * SST_BETA2;
-  double kk = tUc[0], w = tUc[1];
-  double Q0 = P - SST_BSTAR * rho * kk * w;
-  double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
-  double trc[2] = {-(Q0 * V), -(Q1 * V)};
-  for (int i = 0; i < ns; i++) {
-    const int e = fs[i];
-    const T *FF = FFb + (e >= 0 ? e : -e - 1);
-    if (e >= 0) {
-      trc[0] += double(FF[10 * (size_t)nf]);
-      trc[1] += double(FF[11 * (size_t)nf]);
-    } else {
-      trc[0] -= double(FF[10 * (size_t)nf]);
-      trc[1] -= double(FF[11 * (size_t)nf]);
+  if (F::sst) {
+    double tUc[2] = {s.tU[c], s.tU[nc + c]};
+    const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
+    double tgc[4];
+    for (int q = 0; q < 4; q++) tgc[q] = double(TG[q * nc + c]);
+    const double F1c = double(pickc<T>(s.F1, s.F1f)[c]), mutc = s.mut[c];
+    const double rho = s.U[c];
+    double gux = double(G[4 * nc + c]), guy = double(G[5 * nc + c]);
+    double gvx = double(G[6 * nc + c]), gvy = double(G[7 * nc + c]);
+    double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
+    double ddls1 = (guy + gvx) * (guy + gvx);
+    double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
+    double P = (ddls0 + ddls1) * mutc - ddls2;
+    double rdiv = dot2(tgc[0], tgc[1], tgc[2], tgc[3]);
+    double Cw = F1c * SST_CW1 + (1.0 - F1c) * SST_CW2;
+    double beta = F1c * SST_BETA1 + (1.0 - F1c) * SST_BETA2;
+    double kk = tUc[0], w = tUc[1];
+    double Q0 = P - SST_BSTAR * rho * kk * w;
+    double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
+    double trc[2] = {-(Q0 * V), -(Q1 * V)};
+    for (int i = 0; i < ns; i++) {
+      const int e = fs[i];
+      const T *FF = FFb + (e >= 0 ? e : -e - 1);
+      if (e >= 0) {
+        trc[0] += double(FF[10 * (size_t)nf]);
+        trc[1] += double(FF[11 * (size_t)nf]);
+      } else {
+        trc[0] -= double(FF[10 * (size_t)nf]);
+        trc[1] -= double(FF[11 * (size_t)nf]);
+      }
     }
+    s.tR[c] = trc[0];
+    s.tR[nc + c] = trc[1];
+    // SSTkomegaModel::SolveDiag
+    double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
+    s.tD[c] = diagt + 2.0 * SST_BSTAR * w * V;
+    double gradVal = rdiv;
+    s.tD[nc + c] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - F1c) * gradVal / (w * w));
   }
-  s.tR[c] = trc[0];
-  s.tR[nc + c] = trc[1];
-  // SSTkomegaModel::SolveDiag
-  double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
-  s.tD[c] = diagt + 2.0 * SST_BSTAR * w * V;
-  double gradVal = rdiv;
-  s.tD[nc + c] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - F1c) * gradVal / (w * w));
+  // cells without lower neighbours (forward wave 0): dW1 = (-R - 0) / D here,
+  // and the backward update too for cells with no upper neighbour either
+  if (m.wflag[c] & 1) {
+    const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+    double dW1[4], tdW1[2];
+    fwd_store<FLOW>(m, s, c, zero, zero, diag, dW1, tdW1);
+    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -661,26 +759,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
   const double Gamma = USG_GAMMA, OMEGAN = 1.5;
   const double diag = s.D[c];
   const double mprc = F::visc ? s.mprc[c] : 0.0;
-  if constexpr (B && !FWD) {
-    // a batch case whose residual failed a check in this iteration keeps
-    // its state (the other cases are independent of it)
-    if (s.err[2 + m.ccase[c]]) {
-      for (int j = 0; j < 4; j++) {
-        s.dW[j * nc + c] = 0.0;
-        s.Wn[j * nc + c] = s.W[j * nc + c];
-        s.Un[j * nc + c] = s.U[j * nc + c];
-      }
-      if (F::sst) {
-        for (int q = 0; q < 2; q++) {
-          s.tdW[q * nc + c] = 0.0;
-          s.tWn[q * nc + c] = s.tW[q * nc + c];
-          s.tUn[q * nc + c] = s.tU[q * nc + c];
-        }
-        s.mutn[c] = s.mut[c];
-      }
-      return;
-    }
-  }
   const double *dsrc = FWD ? s.dW1 : s.dW;
   const double *tsrc = FWD ? s.tdW1 : s.tdW;
   double dFi[4] = {0, 0, 0, 0};
@@ -734,47 +812,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
       }
     }
   }
+  const double zero[4] = {0.0, 0.0, 0.0, 0.0};
   if (FWD) {
-    for (int j = 0; j < 4; j++) s.dW1[j * nc + c] = (-s.R[j * nc + c] - dFi[j]) / diag;
-    if (F::sst)
-      for (int q = 0; q < 2; q++) s.tdW1[q * nc + c] = (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
+    double dW1[4], tdW1[2];
+    fwd_store<FLOW>(m, s, c, dFi, tFi, diag, dW1, tdW1);
+    // a cell without upper neighbours (backward wave 0) ends here
+    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   } else {
-    double dW[4], Wn[4], Un[4];
-    for (int j = 0; j < 4; j++) {
-      dW[j] = s.dW1[j * nc + c] - dFi[j] / diag;
-      s.dW[j * nc + c] = dW[j];
-      Wn[j] = s.W[j * nc + c] + dW[j];
-    }
-    prim_of(Wn, Un);
-    store4(s.Wn, nc, c, Wn);
-    store4(s.Un, nc, c, Un);
-    if (F::sst) {
-      double tdw[2], tw[2];
-      for (int q = 0; q < 2; q++) {
-        tdw[q] = s.tdW1[q * nc + c] - tFi[q] / s.tD[q * nc + c];
-        s.tdW[q * nc + c] = tdw[q];
-        tw[q] = s.tW[q * nc + c] + tdw[q];
-      }
-      double rho = Un[0];
-      double tu[2] = {tw[0] / rho, tw[1] / rho};
-      // floor at 1e-4 of the free stream (the clip kept commented out at
-      // SSTKWStrategy.h:206-217); a NaN is floored too
-      for (int q = 0; q < 2; q++) {
-        double fl = k.fsnd_turb[q] * 1.0e-4;
-        if (!(tu[q] >= fl)) {
-          tu[q] = fl;
-          tw[q] = rho * fl;
-        }
-      }
-      s.tWn[c] = tw[0];
-      s.tWn[nc + c] = tw[1];
-      s.tUn[c] = tu[0];
-      s.tUn[nc + c] = tu[1];
-      const T *G = pickc<T>(s.grad, s.gradf);
-      double curl = fabs(double(G[6 * nc + c]) - double(G[5 * nc + c]));
-      double h = smax(SST_A1 * tw[1] / rho, double(pickc<T>(s.F2, s.F2f)[c]) * curl);
-      s.mutn[c] = SST_A1 * tw[0] / h;
-    }
+    double dW1[4], tdW1[2] = {0.0, 0.0};
+    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
+    if (F::sst)
+      for (int q = 0; q < 2; q++) tdW1[q] = s.tdW1[q * nc + c];
+    bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, dFi, tFi, diag);
   }
 }
 
@@ -1155,14 +1204,15 @@ template <int FLOW, typename T, bool B>
 static void launch_sweeps(usg_ctx *c) {
   const HostMesh &h = c->hm;
   const int bs = c->blk[USG_K_SWEEP];
+  // wave 0 of each sweep is fused into k_cell / the forward waves (wflag)
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
-  for (int w = 0; w < nw; w++) {
+  for (int w = 1; w < nw; w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
     hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
                        c->k, c->d_fwd, sl);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
-  for (int w = 0; w < nb; w++) {
+  for (int w = 1; w < nb; w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
     hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
                        c->k, c->d_bwd, sl);
@@ -1369,6 +1419,15 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, (double **)&dm.fA, h.fa.data(), nf));
   CK(upload(c, (double **)&dm.fcx, fcx.data(), nf));
   CK(upload(c, (double **)&dm.fcy, fcy.data(), nf));
+  {
+    // bit 0: forward wave 0 (no lower neighbour), bit 1: backward wave 0 (no upper neighbour)
+    std::vector<unsigned char> wflag(nc, 0);
+    if (!h.fwd_poff.empty())
+      for (int i = h.fwd_poff[0]; i < h.fwd_poff[kXcd]; i++) wflag[h.fwd_list[i]] |= 1;
+    if (!h.bwd_poff.empty())
+      for (int i = h.bwd_poff[0]; i < h.bwd_poff[kXcd]; i++) wflag[h.bwd_list[i]] |= 2;
+    CK(upload(c, (unsigned char **)&dm.wflag, wflag.data(), wflag.size()));
+  }
   CK(upload(c, &c->d_fwd, h.fwd_list.data(), nc));
   CK(upload(c, &c->d_bwd, h.bwd_list.data(), nc));
   CK(upload(c, &c->d_cell_list, h.cell_list.data(), nc));
