@@ -29,7 +29,7 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16;  // raw bits
 
 constexpr int kWave = 64;
-constexpr int kChunk = 2048;  // points per partial sum of the weight gradient
+constexpr int kChunk = 256;  // points per partial sum of the weight and bias gradients
 
 thread_local std::string g_err;
 int fail(int code, const std::string &m) {
@@ -211,6 +211,22 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ out, int
   if (threadIdx.x == 0) lossp[blockIdx.x] = sh[0];
 }
 
+// out[slot] = scale * (sum of n partials): one block, thread t sums
+// t, t + 256, ... then a fixed tree (reproducible)
+__global__ __launch_bounds__(256) void k_sum_small(const float *__restrict__ part, int n, float scale,
+                                                   float *__restrict__ out, int slot) {
+  __shared__ double sh[256];
+  double a = 0;
+  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[slot] = (float)(sh[0] * scale);
+}
+
 // Batch assembly: rows idx[start + i] of the resident set (bf16 inputs padded
 // to Kp, fp32 targets), zero rows past n; also the point-blocked input copy.
 __global__ __launch_bounds__(256) void k_gather(const bf16 *__restrict__ xs, int Kp, const float *__restrict__ ys,
@@ -303,6 +319,8 @@ struct usg_mlp {
   float *ys = nullptr;
   int *idx = nullptr;
   long nset = 0, idx_cap = 0;
+  float *lossbuf = nullptr;  // per-step batch losses of an epoch
+  long lossbuf_cap = 0;
 };
 
 namespace usgmlp {
@@ -372,10 +390,10 @@ void forward(usg_mlp *m, int P, bool train) {
       hipLaunchKernelGGL((k_fwd<NT, false, false>), dim3(P / 64, y.N / (NT * 16)), dim3(256), 0, m->s, x, y.K,
                          y.Wt, y.b, y.N, m->Z[l], nullptr, nullptr);
     } else if (train) {
-      hipLaunchKernelGGL((k_fwd<4, true, true>), dim3(P / 64, y.N / 64), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
+      hipLaunchKernelGGL((k_fwd<2, true, true>), dim3(P / 64, y.N / 32), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
                          y.N, m->Z[l], m->H[l], m->Hb[l]);
     } else {
-      hipLaunchKernelGGL((k_fwd<4, true, false>), dim3(P / 64, y.N / 64), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
+      hipLaunchKernelGGL((k_fwd<2, true, false>), dim3(P / 64, y.N / 32), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
                          y.N, nullptr, m->H[l], nullptr);
     }
     x = m->H[l];
@@ -402,7 +420,7 @@ void backward(usg_mlp *m, int P) {
     hipLaunchKernelGGL(k_colsum, dim3(nch), dim3(256), 0, m->s, cur, y.N, P, m->partb);
     hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, m->s, m->partb, nch, (long)y.N, y.gb);
     if (l == 0) break;
-    hipLaunchKernelGGL((k_bwd_data<4>), dim3(P / 64, y.K / 64), dim3(256), 0, m->s, cur, y.N, y.W, y.K, m->Z[l - 1],
+    hipLaunchKernelGGL((k_bwd_data<2>), dim3(P / 64, y.K / 32), dim3(256), 0, m->s, cur, y.N, y.W, y.K, m->Z[l - 1],
                        nxt, nxtb);
     std::swap(cur, nxt);
     std::swap(curb, nxtb);
@@ -430,6 +448,10 @@ int usg_mlp_destroy(usg_mlp *m) {
   if (m->s) hipStreamSynchronize(m->s);
   free_bufs(m);
   for (void *p : m->allocs) hipFree(p);
+  if (m->xs) hipFree(m->xs);
+  if (m->ys) hipFree(m->ys);
+  if (m->idx) hipFree(m->idx);
+  if (m->lossbuf) hipFree(m->lossbuf);
   if (m->s) hipStreamDestroy(m->s);
   delete m;
   return 0;
@@ -573,13 +595,17 @@ int usg_mlp_predict(usg_mlp *m, const float *x, long n, float *y) {
   return 0;
 }
 
-// loss + output gradient + backward of the batch in X/Y (nv valid of P)
-static int loss_and_grads(usg_mlp *m, int P, int nv, float *loss) {
+// loss + output gradient + backward of the batch in X/Y (nv valid of P);
+// with lossbuf, the batch loss goes to lossbuf[slot] on the device (no sync)
+static int loss_and_grads(usg_mlp *m, int P, int nv, float *loss, float *lossbuf = nullptr, int slot = 0) {
   forward(m, P, true);
   const int No = m->L.back().N;
   const int nbl = P * No / 256;
   hipLaunchKernelGGL(k_loss, dim3(nbl), dim3(256), 0, m->s, m->Z.back(), No, m->Y, m->d.nout, nv, m->dA, m->dAb,
                      m->lossp);
+  if (lossbuf)
+    hipLaunchKernelGGL(k_sum_small, dim3(1), dim3(256), 0, m->s, m->lossp, nbl,
+                       (float)(1.0 / ((double)nv * m->d.nout)), lossbuf, slot);
   backward(m, P);
   HIPCHK(hipGetLastError());
   if (loss) {
@@ -655,16 +681,20 @@ int usg_mlp_train_epoch(usg_mlp *m, const int *perm, long n, int batch, float lr
   }
   HIPCHK(hipMemcpyAsync(m->idx, perm, n * sizeof(int), hipMemcpyHostToDevice, m->s));
   const int K0 = m->L[0].K;
-  double lsum = 0;
+  const long nsteps = (n + batch - 1) / batch;
+  if (m->lossbuf_cap < nsteps) {
+    if (m->lossbuf) hipFree(m->lossbuf);
+    m->lossbuf = nullptr;
+    HIPCHK(hipMalloc(&m->lossbuf, nsteps * sizeof(float)));
+    m->lossbuf_cap = nsteps;
+  }
   long nb = 0;
   for (long a = 0; a < n; a += batch) {
     const int nv = (int)std::min<long>(batch, n - a);
     hipLaunchKernelGGL(k_gather, dim3((unsigned)(((long)P * K0 + 255) / 256)), dim3(256), 0, m->s, m->xs, K0, m->ys,
                        m->d.nout, m->idx + a, nv, P, m->X, m->Xb, m->Y);
-    float l = 0;
-    rc = loss_and_grads(m, P, nv, loss_out ? &l : nullptr);
+    rc = loss_and_grads(m, P, nv, nullptr, m->lossbuf, (int)nb);
     if (rc) return rc;
-    lsum += l;
     nb++;
     m->step++;
     const float c1 = 1.0f - std::pow(0.9f, (float)m->step), c2 = 1.0f - std::pow(0.999f, (float)m->step);
@@ -676,7 +706,11 @@ int usg_mlp_train_epoch(usg_mlp *m, const int *perm, long n, int batch, float lr
     }
   }
   HIPCHK(hipGetLastError());
+  std::vector<float> lb(nb);
+  HIPCHK(hipMemcpyAsync(lb.data(), m->lossbuf, nb * sizeof(float), hipMemcpyDeviceToHost, m->s));
   HIPCHK(hipStreamSynchronize(m->s));
+  double lsum = 0;
+  for (float v : lb) lsum += v;
   if (loss_out) *loss_out = (float)(lsum / std::max<long>(nb, 1));
   return 0;
 }
