@@ -163,7 +163,7 @@ def _init(nin, width, nlayer, nout, seed):
     return np.concatenate(w).astype(np.float32), np.concatenate(b).astype(np.float32)
 
 
-def train_net(X, Y, epochs, batch=4096, lr0=1e-3, lr1=2e-5, seed_init=5006, seed_perm=5005):
+def train_net(X, Y, epochs, batch=16384, lr0=2e-3, lr1=2e-5, seed_init=5006, seed_perm=5005):
     net = _mlp.MLP(NIN, 256, 8, NOUT)
     net.set_params(*_init(NIN, 256, 8, NOUT, seed_init))
     net.set_data(X, Y)
@@ -189,7 +189,7 @@ def first_hits(hist: np.ndarray, tol: np.ndarray) -> np.ndarray:
 LEVELS = (1e-4, 1e-6, 1e-8)
 
 
-def experiment(n_train=256, n_test=16, train_iters=20000, epochs=60, max_iters=40000, keep_drop=1e-5):
+def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=40000, keep_drop=1e-5):
     t_all = time.perf_counter()
     train_cases = family_cases(n_train, 5005)
     test_cases = family_cases(n_test, 5007)
@@ -255,7 +255,7 @@ def main():
     ap.add_argument("--train", type=int, default=256)
     ap.add_argument("--test", type=int, default=16)
     ap.add_argument("--train-iters", type=int, default=20000)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--max-iters", type=int, default=40000)
     a = ap.parse_args()
     r = experiment(a.train, a.test, a.train_iters, a.epochs, a.max_iters)
