@@ -199,8 +199,11 @@ void xcd_lists(HostMesh &m) {
   std::stable_sort(key.begin(), key.end());
   m.part.assign(nc, 0);
   for (int r = 0; r < nc; r++) m.part[key[r].second] = (int)((long long)r * kParts / nc);
+  // cell kernels (gradient, cell gather) walk the cells in Hilbert order of
+  // their centres, across colours: a wave's own cells and its neighbours are
+  // then the same few cache lines (the numbering stays colour-blocked)
   std::vector<int> all(nc);
-  for (int i = 0; i < nc; i++) all[i] = i;
+  for (int r = 0; r < nc; r++) all[r] = key[r].second;
   m.cell_off.assign(kParts + 1, 0);
   by_part(all, m.part, m.cell_list, m.cell_off.data());
   std::vector<int> fpart(nf), fall(nf);

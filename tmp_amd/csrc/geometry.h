@@ -35,7 +35,8 @@ struct HostMesh {
   std::vector<int> wall_faces;
   // XCD-aware work lists: cells / faces / wave cells grouped by spatial part
   // (kParts equal slices of the Hilbert curve of the cell centres; a face
-  // belongs to its left cell's part), ascending id inside a part.  Launches
+  // belongs to its left cell's part); cells in Hilbert order, faces and wave
+  // cells in ascending id inside a part.  Launches
   // send part p's blocks to XCD p (blockIdx % kParts) so that neighbour
   // gathers hit the XCD's own L2.
   std::vector<int> part;                      // [nc]

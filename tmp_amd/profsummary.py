@@ -31,7 +31,10 @@ def counters(db: str) -> dict:
     acc: dict = {}
     for name, cn, d, v in rows:
         acc.setdefault(name, {}).setdefault(cn, {})
-        acc[name][cn][d] = acc[name][cn].get(d, 0.0) + v
+        if cn.startswith("GRBM_"):   # whole-GPU counters: the same value per instance, take the max
+            acc[name][cn][d] = max(acc[name][cn].get(d, 0.0), v)
+        else:                        # per-SE / per-channel counters: sum the instances
+            acc[name][cn][d] = acc[name][cn].get(d, 0.0) + v
     return {k: {cn: sum(dv.values()) / len(dv) for cn, dv in cs.items()} for k, cs in acc.items()}
 
 
@@ -70,9 +73,66 @@ def traffic(kernel: str, cells: int, copy_bytes: int, dbs: list, command: str) -
     print(f"traffic_bytes_per_launch: {tb:.6g}")
 
 
+def hbm(trace_db: str, dbs: list, copy_bytes: int, kernels: list, command: str) -> None:
+    """Per kernel: average duration (kernel trace), HBM bytes per dispatch
+    (FETCH_SIZE and WRITE_SIZE, each scaled by its plain-copy calibration)
+    and the achieved GB/s against the 8 TB/s peak."""
+    fetch_db, write_db, cfetch_db, cwrite_db = dbs
+    sf = copy_bytes / (_one(cfetch_db, "k_copy", "FETCH_SIZE") * 1024.0)
+    sw = copy_bytes / (_one(cwrite_db, "k_copy", "WRITE_SIZE") * 1024.0)
+    avg = {name: a for name, n, tot, a, pct in summary(trace_db)}
+    fc, wc = counters(fetch_db), counters(write_db)
+    print("# HBM bytes and achieved bandwidth per kernel\n")
+    if command:
+        print(f"Commands: {command}\n")
+    print(f"FETCH_SIZE scale {sf:.4f}, WRITE_SIZE scale {sw:.4f} (plain copy of {copy_bytes} B).\n")
+    print("| kernel | avg us | read MB | written MB | GB/s | of 8 TB/s |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for k in kernels:
+        names = [n for n in avg if k in n]
+        if not names:
+            continue
+        n = names[0]
+        f = [v["FETCH_SIZE"] for kk, v in fc.items() if k in kk and "FETCH_SIZE" in v]
+        w = [v["WRITE_SIZE"] for kk, v in wc.items() if k in kk and "WRITE_SIZE" in v]
+        if not f or not w:
+            continue
+        rb, wb = f[0] * 1024.0 * sf, w[0] * 1024.0 * sw
+        gbs = (rb + wb) / (avg[n] * 1e-6) / 1e9
+        print(f"| `{n[:70]}` | {avg[n]:.1f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} | {gbs:.0f} | {gbs / 8000:.2f} |")
+
+
+def mfma(trace_db: str, pmc_db: str, command: str, peak_tflops: float = 2500.0) -> None:
+    """Per MFMA kernel: BF16 FLOPs per dispatch (SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512),
+    average duration, TFLOP/s and the MFMA busy fraction
+    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs))."""
+    avg = {name: a for name, n, tot, a, pct in summary(trace_db)}
+    cs = counters(pmc_db)
+    print("# MFMA utilisation of the warm-start MLP kernels\n")
+    if command:
+        print(f"Commands: {command}\n")
+    print("| kernel | avg us | GFLOP per dispatch | TFLOP/s | of 2.5 PFLOP/s | MFMA busy |")
+    print("|---|---:|---:|---:|---:|---:|")
+    for k, v in sorted(cs.items()):
+        if "SQ_INSTS_VALU_MFMA_MOPS_BF16" not in v or v["SQ_INSTS_VALU_MFMA_MOPS_BF16"] <= 0:
+            continue
+        names = [n for n in avg if n.split("(")[0] == k.split("(")[0]]
+        if not names:
+            continue
+        flop = v["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512
+        t = avg[names[0]] * 1e-6
+        busy = v.get("SQ_VALU_MFMA_BUSY_CYCLES", float("nan")) / (v.get("GRBM_GUI_ACTIVE", float("nan")) * 1024)
+        print(f"| `{k[:60]}` | {avg[names[0]]:.1f} | {flop / 1e9:.3f} | {flop / t / 1e12:.1f} | "
+              f"{flop / t / 1e12 / peak_tflops:.3f} | {busy:.3f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("db", nargs="+")
+    ap.add_argument("--hbm", metavar="TRACE_DB", help="achieved GB/s per kernel: db = fetch write copy_fetch copy_write")
+    ap.add_argument("--kernels", nargs="*", default=["k_face", "k_cell", "k_grad", "k_sweep<2, true",
+                                                     "k_sweep<2, false", "k_norm_partial"])
+    ap.add_argument("--mfma", metavar="TRACE_DB", help="MFMA utilisation: db = the --pmc db")
     ap.add_argument("--traffic", metavar="KERNEL")
     ap.add_argument("--cells", type=int, default=0)
     ap.add_argument("--copy-bytes", type=int, default=0)
@@ -80,6 +140,12 @@ def main():
     ap.add_argument("--command", default="")
     ap.add_argument("--pmc", action="store_true", help="the db holds --pmc counters")
     a = ap.parse_args()
+    if a.hbm:
+        hbm(a.hbm, a.db, a.copy_bytes, a.kernels, a.command)
+        return
+    if a.mfma:
+        mfma(a.mfma, a.db[0], a.command)
+        return
     if a.traffic:
         traffic(a.traffic, a.cells, a.copy_bytes, a.db, a.command)
         return
