@@ -136,6 +136,70 @@ __global__ __launch_bounds__(256) void k_bwd_data(const bf16 *__restrict__ dZ, i
 }
 
 // ---------------------------------------------------------------------------
+// Row-major product with the B tile in LDS: C[P][N] = A[P][K] x B, B given
+// transposed (Bt[N][K]).  A block (4 waves) owns 128 points x 64 outputs;
+// each wave 32 points x 64 outputs (2 x 4 MFMA tiles per k step).  The
+// block's 64 rows of Bt are staged in LDS once (rows padded by 8 bf16 so
+// that the 16 lanes reading one column group hit different banks).
+// EPI 0: forward, Z = C + b, H = bf16(silu(Z)) (+ blocked copy and Z when
+// TRAIN); EPI 1: data gradient, dZp = bf16(C * silu'(Zp)) row-major and
+// blocked.
+// ---------------------------------------------------------------------------
+constexpr int kBN = 64;    // outputs per block
+constexpr int kBP = 128;   // points per block
+template <int EPI, bool TRAIN>
+__global__ __launch_bounds__(256) void k_gemm(const bf16 *__restrict__ A, int K, const bf16 *__restrict__ Bt,
+                                              const float *__restrict__ bias, int N, const float *__restrict__ Zp,
+                                              float *__restrict__ Z, bf16 *__restrict__ H, bf16 *__restrict__ Hb) {
+  extern __shared__ __align__(16) bf16 sB[];  // [kBN][K + 8]
+  const int ldb = K + 8;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int n0 = blockIdx.y * kBN;
+  // stage Bt rows n0 .. n0+63 (16-byte chunks)
+  const int chunks = kBN * K / 8;
+  for (int i = threadIdx.x; i < chunks; i += 256) {
+    const int r = i / (K / 8), c8 = i % (K / 8);
+    *reinterpret_cast<bf16x8 *>(&sB[r * ldb + c8 * 8]) = load8(Bt + (size_t)(n0 + r) * K + c8 * 8);
+  }
+  __syncthreads();
+  const int p0 = blockIdx.x * kBP + wave * 32;
+  floatx4 acc[2][4];
+  for (int a = 0; a < 2; a++)
+    for (int t = 0; t < 4; t++) acc[a][t] = floatx4{0, 0, 0, 0};
+  const bf16 *a0p = A + (size_t)(p0 + lane % 16) * K + (lane / 16) * 8;
+  const bf16 *a1p = a0p + (size_t)16 * K;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    const bf16x8 a0 = load8(a0p + k0), a1 = load8(a1p + k0);
+    for (int t = 0; t < 4; t++) {
+      const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(&sB[(t * 16 + lane % 16) * ldb + k0 + (lane / 16) * 8]);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, bv, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, bv, acc[1][t], 0, 0, 0);
+    }
+  }
+  for (int a = 0; a < 2; a++)
+    for (int t = 0; t < 4; t++) {
+      const int n = n0 + t * 16 + lane % 16;
+      for (int r = 0; r < 4; r++) {
+        const int p = p0 + a * 16 + (lane / 16) * 4 + r;
+        const size_t o = (size_t)p * N + n;
+        if (EPI == 0) {
+          const float z = acc[a][t][r] + bias[n];
+          if (TRAIN) Z[o] = z;
+          const bf16 h = to_bf16(z * sigmoid(z));
+          H[o] = h;
+          if (TRAIN) Hb[((size_t)(p / 8) * N + n) * 8 + p % 8] = h;
+        } else {
+          const float z = Zp[o];
+          const float s = sigmoid(z);
+          const bf16 d = to_bf16(acc[a][t][r] * (s * (1.0f + z * (1.0f - s))));
+          H[o] = d;
+          Hb[((size_t)(p / 8) * N + n) * 8 + p % 8] = d;
+        }
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient partial of one chunk of kChunk points:
 // part[chunk][k][n] = sum_{p in chunk} X[p][k] dZ[p][n], 64 k x NT*16 n per
 // block (4 waves x 16 k), operands from the point-blocked copies.
@@ -165,23 +229,45 @@ __global__ __launch_bounds__(256) void k_bwd_weight(const bf16 *__restrict__ Xb,
     }
 }
 
-// bias-gradient partial of one chunk: partb[chunk][n] = sum_p dZ[p][n] (points in order)
-__global__ void k_colsum(const bf16 *__restrict__ dZ, int N, int P, float *__restrict__ partb) {
-  const int n = threadIdx.x;
-  if (n >= N) return;
+// bias-gradient partial of one chunk of kChunk points from the point-blocked
+// copy: block (chunk, 32 columns); thread (column c, j) sums the 8-point
+// blocks j, j + 8, ... of the chunk (16-byte loads), then the 8 partial sums
+// of a column are added in j order (reproducible).
+__global__ __launch_bounds__(256) void k_colsum(const bf16 *__restrict__ dZb, int N, int P, float *__restrict__ partb) {
+  __shared__ float sh[8][32];
+  const int c = threadIdx.x % 32, j = threadIdx.x / 32;
+  const int n = blockIdx.y * 32 + c;
   const int pa = blockIdx.x * kChunk, pb = min(P, pa + kChunk);
   float s = 0;
-  for (int p = pa; p < pb; p++) s += from_bf16(dZ[(size_t)p * N + n]);
-  partb[(size_t)blockIdx.x * N + n] = s;
+  for (int b = pa / 8 + j; b < pb / 8; b += 8) {
+    const bf16x8 v = load8(dZb + ((size_t)b * N + n) * 8);
+    for (int e = 0; e < 8; e++) s += (float)v[e];
+  }
+  sh[j][c] = s;
+  __syncthreads();
+  if (j == 0) {
+    float t = 0;
+    for (int q = 0; q < 8; q++) t += sh[q][c];
+    partb[(size_t)blockIdx.x * N + n] = t;
+  }
 }
 
-// sum of nchunk partials in chunk order
+// sum of nchunk partials in chunk order (loads batched ahead of the adds)
+__device__ __forceinline__ float chunk_sum(const float *__restrict__ part, int nchunk, long len, long i) {
+  float s = 0;
+  int c = 0;
+  for (; c + 8 <= nchunk; c += 8) {
+    float v[8];
+    for (int u = 0; u < 8; u++) v[u] = part[(size_t)(c + u) * len + i];
+    for (int u = 0; u < 8; u++) s += v[u];
+  }
+  for (; c < nchunk; c++) s += part[(size_t)c * len + i];
+  return s;
+}
 __global__ void k_sum_chunks(const float *__restrict__ part, int nchunk, long len, float *__restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
-  float s = 0;
-  for (int c = 0; c < nchunk; c++) s += part[(size_t)c * len + i];
-  out[i] = s;
+  out[i] = chunk_sum(part, nchunk, len, i);
 }
 
 // Loss and output gradient: out[P][No] (fp32), targets y[n][nout] of the
@@ -242,13 +328,14 @@ __global__ __launch_bounds__(256) void k_gather(const bf16 *__restrict__ xs, int
   if (k < nout) Y[(size_t)p * nout + k] = p < nv ? ys[(size_t)idx[p] * nout + k] : 0.0f;
 }
 
-// Adam on the FP32 master weights of one layer (padded K x N), then the BF16
-// copies W[k][n] and Wt[n][k].
-__global__ void k_adam_w(float *w, const float *g, float *m1, float *m2, int K, int N, float lr, float c1, float c2,
-                         bf16 *W, bf16 *Wt) {
+// Adam on one layer (FP32 master weights, padded K x N) straight from the
+// chunk partials of its gradients (summed in chunk order, as k_sum_chunks
+// forms them), then the BF16 copies W[k][n] and Wt[n][k].
+__global__ void k_adam_w_part(float *w, const float *part, int nchunk, float *m1, float *m2, int K, int N, float lr,
+                              float c1, float c2, bf16 *W, bf16 *Wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= K * N) return;
-  const float gi = g[i];
+  const float gi = chunk_sum(part, nchunk, (long)K * N, i);
   const float a = 0.9f * m1[i] + 0.1f * gi;
   const float v = 0.999f * m2[i] + 0.001f * gi * gi;
   m1[i] = a;
@@ -260,10 +347,11 @@ __global__ void k_adam_w(float *w, const float *g, float *m1, float *m2, int K, 
   W[i] = h;
   Wt[(size_t)n * K + k] = h;
 }
-__global__ void k_adam_b(float *b, const float *g, float *m1, float *m2, int n, float lr, float c1, float c2) {
+__global__ void k_adam_b_part(float *b, const float *partb, int nchunk, float *m1, float *m2, int n, float lr,
+                              float c1, float c2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
+  const float gi = chunk_sum(partb, nchunk, n, i);
   const float a = 0.9f * m1[i] + 0.1f * gi;
   const float v = 0.999f * m2[i] + 0.001f * gi * gi;
   m1[i] = a;
@@ -390,23 +478,30 @@ void forward(usg_mlp *m, int P, bool train) {
       hipLaunchKernelGGL((k_fwd<NT, false, false>), dim3(P / 64, y.N / (NT * 16)), dim3(256), 0, m->s, x, y.K,
                          y.Wt, y.b, y.N, m->Z[l], nullptr, nullptr);
     } else if (train) {
-      hipLaunchKernelGGL((k_fwd<2, true, true>), dim3(P / 64, y.N / 32), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
-                         y.N, m->Z[l], m->H[l], m->Hb[l]);
+      hipLaunchKernelGGL((k_gemm<0, true>), dim3(P / kBP, y.N / kBN), dim3(256), kBN * (y.K + 8) * 2, m->s, x, y.K,
+                         y.Wt, y.b, y.N, nullptr, m->Z[l], m->H[l], m->Hb[l]);
     } else {
-      hipLaunchKernelGGL((k_fwd<2, true, false>), dim3(P / 64, y.N / 32), dim3(256), 0, m->s, x, y.K, y.Wt, y.b,
-                         y.N, nullptr, m->H[l], nullptr);
+      hipLaunchKernelGGL((k_gemm<0, false>), dim3(P / kBP, y.N / kBN), dim3(256), kBN * (y.K + 8) * 2, m->s, x, y.K,
+                         y.Wt, y.b, y.N, nullptr, nullptr, m->H[l], nullptr);
     }
     x = m->H[l];
   }
 }
 
-// backward from the output gradient in dA/dAb; gradients into L[l].gw/gb
-void backward(usg_mlp *m, int P) {
+// Adam step parameters (training); null in the gradient-only path
+struct AdamStep {
+  float lr, c1, c2;
+};
+
+// backward from the output gradient in dA/dAb: gradients into L[l].gw/gb,
+// or, with an Adam step, each layer updated right after its last use (the
+// data-gradient product reads W_l before W_l changes)
+void backward(usg_mlp *m, int P, const AdamStep *adam = nullptr) {
   const int nl = (int)m->L.size();
   const int nch = (P + kChunk - 1) / kChunk;
   bf16 *cur = m->dA, *curb = m->dAb, *nxt = m->dB, *nxtb = m->dBb;
   for (int l = nl - 1; l >= 0; l--) {
-    const Layer &y = m->L[l];
+    Layer &y = m->L[l];
     const bf16 *inb = l == 0 ? m->Xb : m->Hb[l - 1];
     // N is 32 (output layer: 2 column tiles of 16) or a multiple of 64 (4 tiles)
     if (y.N % 64 == 0)
@@ -415,13 +510,28 @@ void backward(usg_mlp *m, int P) {
     else
       hipLaunchKernelGGL((k_bwd_weight<2>), dim3(y.K / 64, y.N / 32, nch), dim3(256), 0, m->s, inb, y.K, curb, y.N,
                          P, m->part);
-    hipLaunchKernelGGL(k_sum_chunks, dim3((y.K * y.N + 255) / 256), dim3(256), 0, m->s, m->part, nch,
-                       (long)y.K * y.N, y.gw);
-    hipLaunchKernelGGL(k_colsum, dim3(nch), dim3(256), 0, m->s, cur, y.N, P, m->partb);
-    hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, m->s, m->partb, nch, (long)y.N, y.gb);
+    hipLaunchKernelGGL(k_colsum, dim3(nch, y.N / 32), dim3(256), 0, m->s, curb, y.N, P, m->partb);
+    if (!adam) {
+      hipLaunchKernelGGL(k_sum_chunks, dim3((y.K * y.N + 255) / 256), dim3(256), 0, m->s, m->part, nch,
+                         (long)y.K * y.N, y.gw);
+      hipLaunchKernelGGL(k_sum_chunks, dim3(1), dim3(256), 0, m->s, m->partb, nch, (long)y.N, y.gb);
+    }
+    if (l > 0) {
+      if (y.N % 32 == 0 && y.N >= 64)
+        // dH = dZ W^T: A = dZ [P][N], Bt = W [K][N] (rows = outputs k)
+        hipLaunchKernelGGL((k_gemm<1, true>), dim3(P / kBP, y.K / kBN), dim3(256), kBN * (y.N + 8) * 2, m->s, cur,
+                           y.N, y.W, nullptr, y.K, m->Z[l - 1], nullptr, nxt, nxtb);
+      else
+        hipLaunchKernelGGL((k_bwd_data<2>), dim3(P / 64, y.K / 32), dim3(256), 0, m->s, cur, y.N, y.W, y.K,
+                           m->Z[l - 1], nxt, nxtb);
+    }
+    if (adam) {
+      hipLaunchKernelGGL(k_adam_w_part, dim3((y.K * y.N + 255) / 256), dim3(256), 0, m->s, y.w, m->part, nch, y.m1w,
+                         y.m2w, y.K, y.N, adam->lr, adam->c1, adam->c2, y.W, y.Wt);
+      hipLaunchKernelGGL(k_adam_b_part, dim3((y.N + 255) / 256), dim3(256), 0, m->s, y.b, m->partb, nch, y.m1b, y.m2b,
+                         y.N, adam->lr, adam->c1, adam->c2);
+    }
     if (l == 0) break;
-    hipLaunchKernelGGL((k_bwd_data<2>), dim3(P / 64, y.K / 32), dim3(256), 0, m->s, cur, y.N, y.W, y.K, m->Z[l - 1],
-                       nxt, nxtb);
     std::swap(cur, nxt);
     std::swap(curb, nxtb);
   }
@@ -552,7 +662,7 @@ int usg_mlp_get_params(usg_mlp *m, float *w, float *b) {
 
 // Host batch -> device X (bf16, padded) and Y; returns padded P
 static int upload_batch(usg_mlp *m, const float *x, const float *y, int n, int *Pout) {
-  const int P = round_up(std::max(n, 1), 64);
+  const int P = round_up(std::max(n, 1), 128);
   int rc = ensure_cap(m, P);
   if (rc) return rc;
   const int K0 = m->L[0].K, nin = m->d.nin, nout = m->d.nout;
@@ -597,7 +707,8 @@ int usg_mlp_predict(usg_mlp *m, const float *x, long n, float *y) {
 
 // loss + output gradient + backward of the batch in X/Y (nv valid of P);
 // with lossbuf, the batch loss goes to lossbuf[slot] on the device (no sync)
-static int loss_and_grads(usg_mlp *m, int P, int nv, float *loss, float *lossbuf = nullptr, int slot = 0) {
+static int loss_and_grads(usg_mlp *m, int P, int nv, float *loss, float *lossbuf = nullptr, int slot = 0,
+                          const AdamStep *adam = nullptr) {
   forward(m, P, true);
   const int No = m->L.back().N;
   const int nbl = P * No / 256;
@@ -606,7 +717,7 @@ static int loss_and_grads(usg_mlp *m, int P, int nv, float *loss, float *lossbuf
   if (lossbuf)
     hipLaunchKernelGGL(k_sum_small, dim3(1), dim3(256), 0, m->s, m->lossp, nbl,
                        (float)(1.0 / ((double)nv * m->d.nout)), lossbuf, slot);
-  backward(m, P);
+  backward(m, P, adam);
   HIPCHK(hipGetLastError());
   if (loss) {
     std::vector<float> lp(nbl);
@@ -670,7 +781,7 @@ int usg_mlp_train_epoch(usg_mlp *m, const int *perm, long n, int batch, float lr
   if (n != m->nset) return fail(-1, "perm length differs from the training set");
   for (long i = 0; i < n; i++)
     if (perm[i] < 0 || perm[i] >= n) return fail(-1, "perm entry out of range");
-  const int P = round_up(batch, 64);
+  const int P = round_up(batch, 128);
   int rc = ensure_cap(m, P);
   if (rc) return rc;
   if (m->idx_cap < n) {
@@ -693,17 +804,11 @@ int usg_mlp_train_epoch(usg_mlp *m, const int *perm, long n, int batch, float lr
     const int nv = (int)std::min<long>(batch, n - a);
     hipLaunchKernelGGL(k_gather, dim3((unsigned)(((long)P * K0 + 255) / 256)), dim3(256), 0, m->s, m->xs, K0, m->ys,
                        m->d.nout, m->idx + a, nv, P, m->X, m->Xb, m->Y);
-    rc = loss_and_grads(m, P, nv, nullptr, m->lossbuf, (int)nb);
+    m->step++;
+    const AdamStep st{lr, 1.0f - std::pow(0.9f, (float)m->step), 1.0f - std::pow(0.999f, (float)m->step)};
+    rc = loss_and_grads(m, P, nv, nullptr, m->lossbuf, (int)nb, &st);
     if (rc) return rc;
     nb++;
-    m->step++;
-    const float c1 = 1.0f - std::pow(0.9f, (float)m->step), c2 = 1.0f - std::pow(0.999f, (float)m->step);
-    for (Layer &y : m->L) {
-      hipLaunchKernelGGL(k_adam_w, dim3((y.K * y.N + 255) / 256), dim3(256), 0, m->s, y.w, y.gw, y.m1w, y.m2w, y.K,
-                         y.N, lr, c1, c2, y.W, y.Wt);
-      hipLaunchKernelGGL(k_adam_b, dim3((y.N + 255) / 256), dim3(256), 0, m->s, y.b, y.gb, y.m1b, y.m2b, y.N, lr, c1,
-                         c2);
-    }
   }
   HIPCHK(hipGetLastError());
   std::vector<float> lb(nb);

@@ -104,15 +104,14 @@ def hbm(trace_db: str, dbs: list, copy_bytes: int, kernels: list, command: str) 
 
 def mfma(trace_db: str, pmc_db: str, command: str, peak_tflops: float = 2500.0) -> None:
     """Per MFMA kernel: BF16 FLOPs per dispatch (SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512),
-    average duration, TFLOP/s and the MFMA busy fraction
-    (SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x SIMDs))."""
+    average duration, TFLOP/s and its fraction of the dense BF16 peak."""
     avg = {name: a for name, n, tot, a, pct in summary(trace_db)}
     cs = counters(pmc_db)
     print("# MFMA utilisation of the warm-start MLP kernels\n")
     if command:
         print(f"Commands: {command}\n")
-    print("| kernel | avg us | GFLOP per dispatch | TFLOP/s | of 2.5 PFLOP/s | MFMA busy |")
-    print("|---|---:|---:|---:|---:|---:|")
+    print("| kernel | avg us | GFLOP per dispatch | TFLOP/s | of 2.5 PFLOP/s |")
+    print("|---|---:|---:|---:|---:|")
     for k, v in sorted(cs.items()):
         if "SQ_INSTS_VALU_MFMA_MOPS_BF16" not in v or v["SQ_INSTS_VALU_MFMA_MOPS_BF16"] <= 0:
             continue
@@ -121,9 +120,8 @@ def mfma(trace_db: str, pmc_db: str, command: str, peak_tflops: float = 2500.0) 
             continue
         flop = v["SQ_INSTS_VALU_MFMA_MOPS_BF16"] * 512
         t = avg[names[0]] * 1e-6
-        busy = v.get("SQ_VALU_MFMA_BUSY_CYCLES", float("nan")) / (v.get("GRBM_GUI_ACTIVE", float("nan")) * 1024)
         print(f"| `{k[:60]}` | {avg[names[0]]:.1f} | {flop / 1e9:.3f} | {flop / t / 1e12:.1f} | "
-              f"{flop / t / 1e12 / peak_tflops:.3f} | {busy:.3f} |")
+              f"{flop / t / 1e12 / peak_tflops:.3f} |")
 
 
 def main():
