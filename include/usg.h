@@ -206,6 +206,41 @@ int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, 
  * a quadrilateral.  Host only, no GPU needed. */
 int usg_nodal_weights(int n, const int *nvert, const double *xv, const double *xp, double *w);
 
+/* ---- outputs (host only, no GPU needed) ---------------------------------
+ * U below is a host array of primitive states U[nc*4] = (rho, p, u, v) per
+ * cell, as usg_get_field(USG_F_PRIM) returns it; fs[4] the non-dimensional
+ * free stream (rho, p, u, v). */
+/* Cell vertices in the reference's order (cell3d::pointInd after
+ * Brick4Cobalt2D::insert, CellBrick.cpp:13-136): cnode[nc*4] (-1 padded),
+ * cnv[nc] in {3, 4}. */
+int usg_mesh_cells(const usg_mesh_desc *mesh, int *cnode, int *cnv);
+/* Free stream of usg_ctx_create for a Mach number and angle of attack
+ * (TopologyHolderStrategy::set_fs_variable2, TopologyHolderStrategy.cpp:
+ * 325-396): fs[4] = (rho, p, u, v). */
+int usg_freestream_state(double mach, double aoa_deg, double *fs);
+/* Tecplot FEBLOCK file of node coordinates and cell-centred density,
+ * pressure, velocity, C_p = 2 (p - p_inf) / |u_inf|^2 and the reference's
+ * Mach column |u| / sqrt(p / rho), named "<filename>_<mesh_tag+1>_<proc+1>"
+ * -- TopologyHolderStrategy::writeCellData (TopologyHolderStrategyOutput.cpp:
+ * 466-554), byte for byte. */
+int usg_write_cell_data(const usg_mesh_desc *mesh, const double *U, const double *fs, const char *filename,
+                        int proc, int mesh_tag);
+/* Wall pressure coefficient -- Euler2D::outPutCp (Euler2D.cpp:392-623):
+ * cell pressures averaged to the nodes, C_p = 2 (p - p_inf) / (rho_inf
+ * |u_inf|^2), wall nodes with a signed count (negative where a wall face's
+ * normal has a positive y component) sorted as rankWithXY (non-negative
+ * counts by ascending x, then negative counts by descending x).
+ * table[nwall*4] receives (x, y, count, C_p) of every wall node in that
+ * order (the reference's HDF5 data set); curve[ncurve*4] the same after the
+ * reference's merge of neighbours closer than 1e-4 in x (what its text file
+ * plots).  table and curve hold `cap` rows; pass both NULL to get the counts
+ * only. */
+int usg_surface_cp(const usg_mesh_desc *mesh, const double *U, const double *fs, int cap, double *table,
+                   double *curve, int *nwall, int *ncurve);
+/* The reference's "Cp_<step>" text file (Tecplot POINT zone of x and C_p,
+ * the first point repeated at the end) of a curve from usg_surface_cp. */
+int usg_write_cp(const char *path, const double *curve, int ncurve);
+
 /* Plain device copy kernel (n = bytes/8 doubles) on a stream: the reference
  * kernel of known traffic used to calibrate HBM byte counters. */
 int usg_copy_kernel(void *dst, const void *src, long bytes, void *hip_stream);

@@ -1,0 +1,101 @@
+"""Outputs, monitors and checkpoints of GPU runs.
+
+* A checkpoint (Solver.save / load_checkpoint) taken mid-run and continued
+  in a fresh context gives the same iterates, residual history and forces as
+  the uninterrupted run, bit for bit (the pseudo-time state is U and, for
+  SST, k and omega; everything else is recomputed each iteration).
+* The C++ driver's Loop writes the reference's Tecplot cell-data and Cp
+  files (SolverWrapper.h:71-90); their numbers match the CPU oracle's state
+  at the same step to the files' 6 significant digits.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tmp_amd import usg
+
+from .common import GOLDEN, airfoil_small, part2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("flow", [usg.EULER, usg.SST], ids=["euler", "sst"])
+def test_checkpoint_continues_bit_for_bit(flow, tmp_path):
+    m = airfoil_small()
+    kw = dict(flow=flow, mach=0.15 if flow == usg.SST else 0.5, aoa=3.0, reynolds=6e6)
+    ref = usg.Solver(m, **kw)
+    h_ref = ref.lusgs(12, history=True)
+    a = usg.Solver(m, **kw)
+    a.lusgs(7, history=True)
+    path = str(tmp_path / "state.npz")
+    a.save(path)
+    a.close()
+    b = usg.load_checkpoint(path)
+    assert b.step == 7 and b.hist.shape == (7, 1)
+    b.lusgs(5, history=True)
+    assert b.step == 12
+    assert np.array_equal(b.U, ref.U)
+    if flow == usg.SST:
+        assert np.array_equal(b.field(usg.F_TURB), ref.field(usg.F_TURB))
+    assert np.array_equal(b.hist[:, 0], h_ref)
+    assert np.array_equal(b.hist_step, np.arange(1, 13))
+    assert b.forces() == ref.forces()
+
+
+def test_batch_checkpoint(tmp_path):
+    meshes = [airfoil_small(), part2()]
+    kw = dict(flow=usg.EULER, mach=[0.5, 0.4], aoa=[2.0, 4.0])
+    ref = usg.BatchSolver(meshes, **kw)
+    ref.lusgs(6)
+    a = usg.BatchSolver(meshes, **kw)
+    a.lusgs(4)
+    a.save(str(tmp_path / "b.npz"))
+    b = usg.load_checkpoint(str(tmp_path / "b.npz"))
+    assert isinstance(b, usg.BatchSolver) and b.ncase == 2
+    b.lusgs(2)
+    assert np.array_equal(b.U, ref.U)
+    cl, cd = b.forces()
+    rcl, rcd = ref.forces()
+    assert np.array_equal(cl, rcl) and np.array_equal(cd, rcd)
+
+
+def _numbers(path, skip):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    return lines[:skip], lines[skip:]
+
+
+def test_driver_writes_reference_outputs(tmp_path):
+    exe = os.path.join(os.path.dirname(usg.LIB_PATH), "usg_run")
+    out = subprocess.run([exe, os.path.join(GOLDEN, "part2.grd"), "-2", "-2147483647", "Euler2D", "LUSGS",
+                          "0.5", "1e6", "2.0", "6", "1e-30", "5"], capture_output=True, text=True, timeout=120,
+                         cwd=str(tmp_path))
+    assert out.returncode == 0, out.stderr
+    # dumps after steps 0 and 5 (i % 5 == 0): states after 1 and 6 iterations
+    for name in ("CellDataParallelLUSGS1_1_1", "CellDataParallelLUSGS6_1_1", "Cp_0", "Cp_5"):
+        assert os.path.exists(tmp_path / name), name
+    m = part2()
+    orc = O.OracleSolver(O.OracleMesh(m), O.EULER, 0.5, 2.0, 1e6)
+    fs = np.array(list(orc.s.fs_primVar)[:4])
+    for _ in range(6):
+        orc.lusgs_step()
+    want = usg.write_cell_data(str(tmp_path / "oracle"), m, orc.U, fs)
+    h1, v1 = _numbers(tmp_path / "CellDataParallelLUSGS6_1_1", 4)
+    h2, v2 = _numbers(want, 4)
+    assert h1 == h2
+    nn, nc = m.nn, m.nc
+    a = np.array([float(x) for x in v1[:2 * nn + 6 * nc]])
+    b = np.array([float(x) for x in v2[:2 * nn + 6 * nc]])
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-9)
+    assert v1[2 * nn + 6 * nc:] == v2[2 * nn + 6 * nc:]          # connectivity
+    _, curve = usg.surface_cp(m, orc.U, fs)
+    usg.write_cp(str(tmp_path / "oracle_cp"), curve)
+    h1, v1 = _numbers(tmp_path / "Cp_5", 3)
+    h2, v2 = _numbers(tmp_path / "oracle_cp", 3)
+    assert h1 == h2 and len(v1) == len(v2) == curve.shape[0] + 1
+    a = np.array([[float(t) for t in l.split("\t")] for l in v1])
+    b = np.array([[float(t) for t in l.split("\t")] for l in v2])
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-9)

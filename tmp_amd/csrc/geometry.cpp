@@ -245,6 +245,47 @@ void xcd_lists(HostMesh &m) {
 
 }  // namespace
 
+std::string cell_vertices(int nn, int nf, const int *fnode, const int *flc, const int *frc, int nc,
+                          std::vector<int> &cnode, std::vector<int> &cnv) {
+  if (nn <= 0 || nf <= 0 || nc <= 0 || !fnode || !flc || !frc) return "empty mesh";
+  for (int k = 0; k < nf; k++) {
+    if (flc[k] < 0 || flc[k] >= nc) return "Left cell can't be negative";
+    if (frc[k] >= nc || (frc[k] < 0 && frc[k] != kWall && frc[k] != kFarfield))
+      return "Not a valid boundary type";
+    if (fnode[2 * k] < 0 || fnode[2 * k] >= nn || fnode[2 * k + 1] < 0 || fnode[2 * k + 1] >= nn)
+      return "face node out of range";
+  }
+  std::vector<Brick> br(nc);
+  for (int k = 0; k < nf; k++) {
+    br[flc[k]].insert(&fnode[2 * k]);
+    if (frc[k] >= 0) br[frc[k]].insert(&fnode[2 * k]);
+  }
+  cnode.assign(4 * (size_t)nc, -1);
+  cnv.assign(nc, 0);
+  for (int i = 0; i < nc; i++) {
+    int nv = br[i].size;
+    if (nv < 3 || nv > 4) return "cell is not a triangle or quadrilateral";
+    cnv[i] = nv;
+    for (int v = 0; v < nv; v++) cnode[4 * i + v] = br[i].nodes[v];
+  }
+  return "";
+}
+
+void freestream_state(double mach, double aoa_deg, double fs[4], double *aoa_rad) {
+  const double Gamma = 1.4, PI = 3.141593;
+  double fs_AOA = PI * aoa_deg / 180;
+  double fs_density = 1.225, fs_pressure = 101325;
+  double fs_soundSpeed = std::sqrt(Gamma * fs_pressure / fs_density);
+  double fs_velocity_magnitude = fs_soundSpeed * mach;
+  double u = fs_velocity_magnitude * std::cos(fs_AOA);
+  double v = fs_velocity_magnitude * std::sin(fs_AOA);
+  fs[0] = fs_density / fs_density;
+  fs[1] = fs_pressure / fs_pressure;
+  fs[2] = u / (fs_soundSpeed / std::sqrt(Gamma));
+  fs[3] = v / (fs_soundSpeed / std::sqrt(Gamma));
+  if (aoa_rad) *aoa_rad = fs_AOA;
+}
+
 std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,
                        const int *flc, const int *frc, int nc) {
   if (nn <= 0 || nf <= 0 || nc <= 0 || !xy || !fnode || !flc || !frc) return "empty mesh";
@@ -255,22 +296,10 @@ std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int 
   m.fnode.assign(fnode, fnode + 2 * (size_t)nf);
   m.flc.assign(flc, flc + nf);
   m.frc.assign(frc, frc + nf);
-  for (int k = 0; k < nf; k++) {
-    if (flc[k] < 0 || flc[k] >= nc) return "Left cell can't be negative";
-    if (frc[k] >= nc || (frc[k] < 0 && frc[k] != kWall && frc[k] != kFarfield))
-      return "Not a valid boundary type";
-    if (fnode[2 * k] < 0 || fnode[2 * k] >= nn || fnode[2 * k + 1] < 0 || fnode[2 * k + 1] >= nn)
-      return "face node out of range";
-  }
-  m.cnode.assign(4 * (size_t)nc, -1);
-  m.cnv.assign(nc, 0);
+  std::string e = cell_vertices(nn, nf, fnode, flc, frc, nc, m.cnode, m.cnv);
+  if (!e.empty()) return e;
   m.cface.assign(4 * (size_t)nc, kNoFace);
   m.cnf.assign(nc, 0);
-  std::vector<Brick> br(nc);
-  for (int k = 0; k < nf; k++) {
-    br[flc[k]].insert(&fnode[2 * k]);
-    if (frc[k] >= 0) br[frc[k]].insert(&fnode[2 * k]);
-  }
   for (int k = 0; k < nf; k++) {
     int l = flc[k], r = frc[k];
     if (m.cnf[l] >= 4 || (r >= 0 && m.cnf[r] >= 4)) return "cell with more than 4 faces";
@@ -280,14 +309,12 @@ std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int 
   m.vol.assign(nc, 0.0);
   m.cc.assign(2 * (size_t)nc, 0.0);
   for (int i = 0; i < nc; i++) {
-    int nv = br[i].size;
-    if (nv < 3 || nv > 4 || m.cnf[i] != nv) return "cell is not a triangle or quadrilateral";
-    m.cnv[i] = nv;
+    int nv = m.cnv[i];
+    if (m.cnf[i] != nv) return "cell is not a triangle or quadrilateral";
     double xv[4][2];
     for (int v = 0; v < nv; v++) {
-      m.cnode[4 * i + v] = br[i].nodes[v];
-      xv[v][0] = xy[2 * br[i].nodes[v]];
-      xv[v][1] = xy[2 * br[i].nodes[v] + 1];
+      xv[v][0] = xy[2 * m.cnode[4 * i + v]];
+      xv[v][1] = xy[2 * m.cnode[4 * i + v] + 1];
     }
     m.vol[i] = volume(xv, nv);
     double sx = 0, sy = 0;

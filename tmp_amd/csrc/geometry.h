@@ -52,6 +52,21 @@ struct HostMesh {
 
 constexpr int kParts = 8;  // XCDs of an MI355X
 
+// Validates the faces and returns each cell's vertices in the reference's
+// order (Brick4Cobalt2D::insert, CellBrick.cpp:13-136): cnode[nc*4] (-1
+// padded), cnv[nc] in {3, 4}.  Returns an empty string or the error text.
+std::string cell_vertices(int nn, int nf, const int *fnode, const int *flc, const int *frc, int nc,
+                          std::vector<int> &cnode, std::vector<int> &cnv);
+
+// Non-dimensional free stream (rho, p, u, v) of Mach `mach` at `aoa_deg`,
+// as TopologyHolderStrategy::set_fs_variable2 computes it (sea-level
+// density and pressure, pi = 3.141593); aoa_rad (may be null) gets the
+// angle in radians.
+void freestream_state(double mach, double aoa_deg, double fs[4], double *aoa_rad);
+
+// Sets the calling thread's error text (usg_last_error) and returns code.
+int set_error(int code, const std::string &msg);
+
 // Validates and copies the mesh, then computes the derived geometry.
 // Returns an empty string on success, else the error text.
 std::string build_mesh(HostMesh &m, int nn, const double *xy, int nf, const int *fnode,

@@ -1023,6 +1023,10 @@ static int fail(int code, const std::string &msg) {
   return code;
 }
 
+namespace usg {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+}  // namespace usg
+
 #define HIPCHK(x)                                                                          \
   do {                                                                                     \
     hipError_t e_ = (x);                                                                   \
@@ -1095,20 +1099,14 @@ static int upload(usg_ctx *c, T **p, const T *src, size_t n) {
 // set_fs_variable2 + update_nondim_variable (TopologyHolderStrategy.cpp:325-396)
 // with WrapSolver's density 1.225 and pressure 101325 (SolverWrapper.h:55).
 static void freestream(const usg_params &prm, Consts &kk, double &aoa_rad) {
-  const double R = 287.0, Gamma = 1.4, PI = 3.141593;
+  const double R = 287.0, Gamma = 1.4;
   double fs_mach = prm.mach;
-  double fs_AOA = PI * prm.aoa_deg / 180;
   double fs_density = 1.225, fs_pressure = 101325, fs_Re = prm.reynolds;
   double fs_soundSpeed = std::sqrt(Gamma * fs_pressure / fs_density);
   double fs_Temperature = fs_pressure / (R * fs_density);
-  double fs_velocity_magnitude = fs_soundSpeed * fs_mach;
   double fs_mu = 1.458e-6 * std::pow(fs_Temperature, 1.5) / (fs_Temperature + 110.5);
-  double u = fs_velocity_magnitude * std::cos(fs_AOA);
-  double v = fs_velocity_magnitude * std::sin(fs_AOA);
-  kk.fs[0] = fs_density / fs_density;
-  kk.fs[1] = fs_pressure / fs_pressure;
-  kk.fs[2] = u / (fs_soundSpeed / std::sqrt(Gamma));
-  kk.fs[3] = v / (fs_soundSpeed / std::sqrt(Gamma));
+  double fs_AOA;
+  freestream_state(prm.mach, prm.aoa_deg, kk.fs, &fs_AOA);
   kk.fs_c = std::sqrt(Gamma);
   kk.sqrt_gamma = std::sqrt(Gamma);
   kk.mach = fs_mach;

@@ -1,10 +1,12 @@
 // Command-line driver over the C++ adapters, in the shape of the reference's
 // serial scenarios (src/main.cpp:588-697, unstruct_serial): read a Cobalt
 // mesh, set the flow, run the integrator, print the residual history and
-// the wall force.
+// the wall force.  With outputEvery > 0 the loop writes the reference's
+// Tecplot cell data and Cp files into the working directory every
+// outputEvery steps (the reference: 5000).
 //
 //   usg_run <mesh.grd> <wallTag> <farTag> <Euler2D|ViscousFlow2D|SST2D>
-//           <LUSGS|RungeKutta5> <Ma> <Re> <AoA> <maxStep> [minRes]
+//           <LUSGS|RungeKutta5> <Ma> <Re> <AoA> <maxStep> [minRes] [outputEvery]
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
@@ -16,7 +18,7 @@ int main(int argc, char **argv) {
   if (argc < 10) {
     std::fprintf(stderr,
                  "usage: %s mesh.grd wallTag farTag Euler2D|ViscousFlow2D|SST2D LUSGS|RungeKutta5 Ma Re AoA "
-                 "maxStep [minRes]\n",
+                 "maxStep [minRes] [outputEvery]\n",
                  argv[0]);
     return 2;
   }
@@ -33,6 +35,7 @@ int main(int argc, char **argv) {
     w.setTopology(&rd);
     w.setParams(argv[4], argv[5], "STABLE", std::atoi(argv[9]), minres, std::atof(argv[6]), std::atof(argv[7]),
                 &aoa);
+    w.setOutputEvery(argc > 11 ? std::atoi(argv[11]) : 0);
     w.init();
     int n = w.Loop(std::max(1, std::atoi(argv[9]) / 10));
     double cl, cd;

@@ -125,6 +125,7 @@ class GpuFlowStrategy {
       check(usg_ctx_create(&d, &p, &c));
       ctx_.push_back(c);
       nc_.push_back(d.nc);
+      meshes_.push_back(rd.mesh(i));
     }
     mirrors_.assign(16, std::vector<std::vector<double>>(ctx_.size()));
     ptrs_.assign(16, std::vector<double *>(ctx_.size(), nullptr));
@@ -172,6 +173,40 @@ class GpuFlowStrategy {
   }
   usg_ctx *context(int m) { return ctx_.at(m); }
 
+  // TopologyHolderStrategy::writeCellData (TopologyHolderStrategyOutput.cpp:
+  // 466-554): Tecplot file "<filename>_<meshTag+1>_<proc+1>" of the current
+  // primitive state of mesh meshTag
+  void writeCellData(const std::string &filename, int proc, int meshTag) {
+    double **U = getU();
+    usg_mesh_desc d = meshes_.at(meshTag).desc();
+    double fs[4];
+    check(usg_freestream_state(ma_, aoa_, fs));
+    check(usg_write_cell_data(&d, U[meshTag], fs, filename.c_str(), proc, meshTag));
+  }
+  // Wall C_p of mesh mesh_ind (Euler2D::outPutCp, Euler2D.cpp:392-623):
+  // table = every wall node (x, y, signed count, C_p) in the reference's
+  // order, curve = the merged rows its text file plots
+  void surfaceCp(int mesh_ind, std::vector<double> &table, std::vector<double> &curve) {
+    double **U = getU();
+    usg_mesh_desc d = meshes_.at(mesh_ind).desc();
+    double fs[4];
+    check(usg_freestream_state(ma_, aoa_, fs));
+    int nw = 0, ncv = 0;
+    check(usg_surface_cp(&d, U[mesh_ind], fs, 0, nullptr, nullptr, &nw, &ncv));
+    table.assign(4 * (size_t)nw, 0.0);
+    curve.assign(4 * (size_t)nw, 0.0);
+    check(usg_surface_cp(&d, U[mesh_ind], fs, nw, table.data(), curve.data(), &nw, &ncv));
+    curve.resize(4 * (size_t)ncv);
+  }
+  // Euler2D::outPutCp: writes the text file "Cp_<step>" in the working
+  // directory.  The reference also stores the sorted table in the HDF5 file
+  // `filename`; there is no HDF5 library here, surfaceCp returns that table.
+  void outPutCp(const std::string & /*filename*/, int mesh_ind, int step) {
+    std::vector<double> table, curve;
+    surfaceCp(mesh_ind, table, curve);
+    check(usg_write_cp(("Cp_" + std::to_string(step)).c_str(), curve.data(), (int)(curve.size() / 4)));
+  }
+
  private:
   double **mirror(int field, int width) {
     for (size_t m = 0; m < ctx_.size(); m++) {
@@ -186,6 +221,7 @@ class GpuFlowStrategy {
   double ma_ = 0.5, aoa_ = 0.0, re_ = 1e6, cfl_lusgs_ = 400, cfl_rk_ = 0.8;
   std::vector<usg_ctx *> ctx_;
   std::vector<int> nc_;
+  std::vector<MeshData> meshes_;
   std::vector<std::vector<std::vector<double>>> mirrors_;
   std::vector<std::vector<double *>> ptrs_;
 };
@@ -273,14 +309,22 @@ class WrapSolver {
     advance_->initializeStrategy(flow_.get());
     advance_->initializeData();
   }
-  // Loop (SolverWrapper.h:71-90) without the Tecplot dumps; stops at
-  // MaximumTimeStep or when the relative residual reaches MinimumResidual.
+  // Loop (SolverWrapper.h:71-90): after step i with i % outputEvery == 0
+  // every mesh j writes "CellDataParallelLUSGS<i+1>_<j+1>_1" (writeCellData)
+  // and "Cp_<i>" (outPutCp), as the reference does every 5000 steps; stops
+  // at MaximumTimeStep or when the relative residual reaches MinimumResidual.
   int Loop(int report_every = 0) {
     int i = 0;
     for (; i < maxStep_; i++) {
       advance_->singleStep(i);
       if (report_every > 0 && i % report_every == 0)
         std::cout << i << " " << advance_->lastResidual() << '\n';
+      if (outputEvery_ > 0 && i % outputEvery_ == 0) {
+        for (int j = 0; j < flow_->getNMesh(); j++) {
+          flow_->writeCellData("CellDataParallelLUSGS" + std::to_string(i + 1), 0, j);
+          flow_->outPutCp("OneAndOnlyLegendaryCp_" + std::to_string(j + 1), j, i);
+        }
+      }
       if (advance_->converged()) {
         i++;
         break;
@@ -290,6 +334,8 @@ class WrapSolver {
   }
   GpuFlowStrategy *flow() { return flow_.get(); }
   GpuTimeIntegrator *advance() { return advance_.get(); }
+  // steps between output dumps in Loop (the reference's 5000); 0 = none
+  void setOutputEvery(int n) { outputEvery_ = n; }
 
  private:
   UnstructReader *reader_ = nullptr;
@@ -297,6 +343,7 @@ class WrapSolver {
   std::unique_ptr<GpuTimeIntegrator> advance_;
   int maxStep_ = 0;
   double minRes_ = 0;
+  int outputEvery_ = 5000;
 };
 
 }  // namespace usg

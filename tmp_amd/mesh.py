@@ -389,13 +389,16 @@ def channel_bump(ni: int = 160, nj: int = 70, nq: int = 8, seed: int = 1001,
     return m
 
 
-def naca4(t: float, m_camber: float = 0.0, p: float = 0.4, n: int = 256) -> np.ndarray:
+def naca4(t: float, m_camber: float = 0.0, p: float = 0.4, n: int = 256, closed_te: bool = False) -> np.ndarray:
     """Closed contour of a 4-digit airfoil (chord 1), counter-clockwise from
-    the upper trailing-edge point; the blunt trailing-edge base closes it.  Thickness y_t = 5t(0.2969 sqrt(x) - 0.1260x - 0.3516x^2
-    + 0.2843x^3 - 0.1015x^4)."""
+    the upper trailing-edge point.  Thickness y_t = 5t(0.2969 sqrt(x) -
+    0.1260x - 0.3516x^2 + 0.2843x^3 - 0.1015x^4): a blunt trailing edge whose
+    base closes the contour; with ``closed_te`` the last coefficient is
+    -0.1036, the variant of the formula with a sharp trailing edge."""
     beta = np.linspace(0, np.pi, n // 2 + 1)
     x = 0.5 * (1 - np.cos(beta))
-    yt = 5 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 - 0.1015 * x ** 4)
+    a4 = -0.1036 if closed_te else -0.1015
+    yt = 5 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3 + a4 * x ** 4)
     if m_camber > 0:
         yc = np.where(x < p, m_camber / p ** 2 * (2 * p * x - x ** 2),
                       m_camber / (1 - p) ** 2 * ((1 - 2 * p) + 2 * p * x - x ** 2))
@@ -415,7 +418,8 @@ def naca4(t: float, m_camber: float = 0.0, p: float = 0.4, n: int = 256) -> np.n
 
 def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: float = 4.5e-6,
                    rfar: float = 50.0, growth: float = 1.15, seed: int | None = None,
-                   jitter: float = 0.0, quad_only: bool = False, blend_r: float = 1.0) -> Mesh:
+                   jitter: float = 0.0, quad_only: bool = False, blend_r: float = 1.0,
+                   closed_te: bool = False) -> Mesh:
     """Hybrid O-mesh around a 4-digit airfoil (chord 1, far field at ``rfar``).
 
     Layers leave the wall with first spacing ``dy1`` growing by ``growth``
@@ -430,7 +434,7 @@ def airfoil_hybrid(ni: int = 192, t: float = 0.12, camber: float = 0.0, dy1: flo
     same angle, which cannot fold the way long marches from a concave
     (cambered) surface do.
     """
-    c = naca4(t, camber, n=ni)
+    c = naca4(t, camber, n=ni, closed_te=closed_te)
     c = c[:-1] if np.allclose(c[0], c[-1]) else c
     # resample to exactly ni points around (contour is closed through the TE base)
     seg = np.linalg.norm(np.diff(np.vstack([c, c[:1]]), axis=0), axis=1)

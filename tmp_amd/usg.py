@@ -54,7 +54,8 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
-           "usg_device_arch", "usg_case_status", "usg_nodal_weights"]
+           "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
+           "usg_write_cell_data", "usg_surface_cp", "usg_write_cp"]
 
 _lib = None
 
@@ -95,6 +96,11 @@ def lib():
         L.usg_autotune.argtypes = [vp, C.c_int, _ip]
         L.usg_device_arch.argtypes = [C.c_char_p, C.c_int]
         L.usg_case_status.argtypes = [vp, _ip]
+        L.usg_mesh_cells.argtypes = [C.POINTER(_MeshDesc), _ip, _ip]
+        L.usg_freestream_state.argtypes = [C.c_double, C.c_double, _dp]
+        L.usg_write_cell_data.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_char_p, C.c_int, C.c_int]
+        L.usg_surface_cp.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_int, _dp, _dp, _ip, _ip]
+        L.usg_write_cp.argtypes = [C.c_char_p, _dp, C.c_int]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -123,6 +129,59 @@ def mesh_geometry(m: _mesh.Mesh) -> dict:
                                               ("vol", "cc", "fn", "fa", "fc", "wdist")), C.byref(a), C.byref(b)))
     out["nfwd"], out["nbwd"] = a.value, b.value
     return out
+
+
+def cell_nodes(m: _mesh.Mesh) -> tuple[np.ndarray, np.ndarray]:
+    """Cell vertices in the reference's order: (cnode[nc, 4] -1 padded, cnv[nc])."""
+    d, keep = _desc(m)
+    cnode = np.zeros((m.nc, 4), np.int32)
+    cnv = np.zeros(m.nc, np.int32)
+    _check(lib().usg_mesh_cells(C.byref(d), cnode.ctypes.data_as(_ip), cnv.ctypes.data_as(_ip)))
+    return cnode, cnv
+
+
+def freestream_state(mach: float, aoa: float) -> np.ndarray:
+    """Non-dimensional free stream (rho, p, u, v) of a context at Mach `mach`, `aoa` degrees."""
+    fs = np.zeros(4)
+    _check(lib().usg_freestream_state(float(mach), float(aoa), fs.ctypes.data_as(_dp)))
+    return fs
+
+
+def _prim_fs(m, U, fs):
+    U = np.ascontiguousarray(U, np.float64)
+    if U.shape != (m.nc, 4):
+        raise UsgError(f"U must be ({m.nc}, 4), got {U.shape}")
+    return U, np.ascontiguousarray(fs, np.float64).reshape(4)
+
+
+def write_cell_data(filename: str, m: _mesh.Mesh, U, fs, proc: int = 0, mesh_tag: int = 0) -> str:
+    """writeCellData: Tecplot file "<filename>_<mesh_tag+1>_<proc+1>"; returns its path."""
+    U, fs = _prim_fs(m, U, fs)
+    d, keep = _desc(m)
+    _check(lib().usg_write_cell_data(C.byref(d), U.ctypes.data_as(_dp), fs.ctypes.data_as(_dp),
+                                     os.fsencode(filename), proc, mesh_tag))
+    return f"{filename}_{mesh_tag + 1}_{proc + 1}"
+
+
+def surface_cp(m: _mesh.Mesh, U, fs) -> tuple[np.ndarray, np.ndarray]:
+    """outPutCp: (table[nwall, 4], curve[ncurve, 4]) of (x, y, signed count, Cp)."""
+    U, fs = _prim_fs(m, U, fs)
+    d, keep = _desc(m)
+    nw, ncv = C.c_int(), C.c_int()
+    L = lib()
+    _check(L.usg_surface_cp(C.byref(d), U.ctypes.data_as(_dp), fs.ctypes.data_as(_dp), 0, None, None,
+                            C.byref(nw), C.byref(ncv)))
+    table = np.zeros((nw.value, 4))
+    curve = np.zeros((nw.value, 4))
+    _check(L.usg_surface_cp(C.byref(d), U.ctypes.data_as(_dp), fs.ctypes.data_as(_dp), nw.value,
+                            table.ctypes.data_as(_dp), curve.ctypes.data_as(_dp), C.byref(nw), C.byref(ncv)))
+    return table, curve[:ncv.value].copy()
+
+
+def write_cp(path: str, curve) -> None:
+    """The reference's Cp_<step> text file of a curve from surface_cp."""
+    c = np.ascontiguousarray(curve, np.float64).reshape(-1, 4)
+    _check(lib().usg_write_cp(os.fsencode(path), c.ctypes.data_as(_dp), c.shape[0]))
 
 
 ROW, COLUMN = 0, 1
@@ -165,6 +224,16 @@ class Solver:
         h = C.c_void_p()
         _check(lib().usg_ctx_create(C.byref(d), C.byref(p), C.byref(h)))
         self._h = h
+        self.meshes = [m]
+        self.case_params = [dict(mach=float(mach), aoa=float(aoa), reynolds=float(reynolds))]
+        self._common(flow, precision, cfl_lusgs, cfl_rk)
+
+    def _common(self, flow, precision, cfl_lusgs, cfl_rk):
+        self.precision = precision
+        self.cfl = (float(cfl_lusgs), float(cfl_rk))
+        self.step = 0                    # pseudo-time iterations taken
+        self.hist_step = np.zeros(0, np.int64)          # iterations with a recorded residual
+        self.hist = np.zeros((0, self.ncase))           # their residual norms per case
 
     def close(self):
         if getattr(self, "_h", None):
@@ -216,9 +285,13 @@ class Solver:
     def _steps(self, fn, n, history):
         h = np.zeros(max(n, 1) * self.ncase) if history else None
         _check(fn(self._h, n, h.ctypes.data_as(_dp) if history else None))
+        first = self.step + 1
+        self.step += n
         if not history:
             return None
         h = h[:n * self.ncase].reshape(n, self.ncase)
+        self.hist_step = np.concatenate([self.hist_step, np.arange(first, first + n)])
+        self.hist = np.concatenate([self.hist, h])
         return h if isinstance(self, BatchSolver) else h[:, 0]
 
     def lusgs(self, n: int = 1, history: bool = False):
@@ -259,6 +332,47 @@ class Solver:
         _check(lib().usg_kernel_time(self._h, C.byref(t), C.byref(n)))
         return t.value, n.value
 
+    # --- outputs and checkpoints ----------------------------------------
+    def freestream(self, case: int = 0) -> np.ndarray:
+        p = self.case_params[case]
+        return freestream_state(p["mach"], p["aoa"])
+
+    def case_U(self, case: int = 0) -> np.ndarray:
+        c = self.case_cell
+        return self.U[int(c[case]):int(c[case + 1])]
+
+    def write_cell_data(self, filename: str, proc: int = 0, mesh_tag: int = 0, case: int = 0) -> str:
+        """writeCellData of one case's current state; returns the file path."""
+        return write_cell_data(filename, self.meshes[case], self.case_U(case), self.freestream(case), proc,
+                               mesh_tag)
+
+    def surface_cp(self, case: int = 0):
+        """outPutCp of one case's current state: (table, curve), see surface_cp."""
+        return surface_cp(self.meshes[case], self.case_U(case), self.freestream(case))
+
+    def write_cp(self, path: str, case: int = 0) -> np.ndarray:
+        _, curve = self.surface_cp(case)
+        write_cp(path, curve)
+        return curve
+
+    def save(self, path: str) -> None:
+        """Checkpoint: meshes, parameters, state (U, and k, omega for SST),
+        iteration count and residual history, in one .npz file."""
+        d = {"flow": self.flow, "precision": self.precision, "cfl": np.array(self.cfl), "step": self.step,
+             "ncase": self.ncase, "batch": isinstance(self, BatchSolver),
+             "mach": np.array([p["mach"] for p in self.case_params]),
+             "aoa": np.array([p["aoa"] for p in self.case_params]),
+             "reynolds": np.array([p["reynolds"] for p in self.case_params]),
+             "U": self.U, "hist_step": self.hist_step, "hist": self.hist}
+        if self.flow == SST:
+            d["turb"] = self.field(F_TURB)
+        for i, m in enumerate(self.meshes):
+            d.update({f"xy{i}": m.xy, f"fnode{i}": m.fnode, f"flc{i}": m.flc, f"frc{i}": m.frc,
+                      f"nc{i}": m.nc, f"name{i}": m.name})
+        with open(path, "wb") as f:
+            np.savez(f, **d)
+
+
     # --- launch configuration -------------------------------------------
     def blocks(self) -> list:
         out = []
@@ -276,6 +390,29 @@ class Solver:
         best = np.zeros(4, np.int32)
         _check(lib().usg_autotune(self._h, iters, best.ctypes.data_as(_ip)))
         return best.tolist()
+
+
+def load_checkpoint(path: str, precision: int | None = None) -> "Solver":
+    """A Solver (or BatchSolver) restored from Solver.save: same meshes and
+    parameters, state loaded with usg_set_prim, iteration count and history
+    kept.  Continuing it gives the same iterates as the run that saved it."""
+    with np.load(path, allow_pickle=False) as z:
+        k = int(z["ncase"])
+        meshes = [_mesh.Mesh(z[f"xy{i}"], z[f"fnode{i}"], z[f"flc{i}"], z[f"frc{i}"], int(z[f"nc{i}"]),
+                             str(z[f"name{i}"])) for i in range(k)]
+        flow = int(z["flow"])
+        prec = int(z["precision"]) if precision is None else precision
+        cfl = z["cfl"]
+        if bool(z["batch"]):
+            s = BatchSolver(meshes, flow, z["mach"], z["aoa"], z["reynolds"], prec, cfl[0], cfl[1])
+        else:
+            s = Solver(meshes[0], flow, float(z["mach"][0]), float(z["aoa"][0]), float(z["reynolds"][0]), prec,
+                       cfl[0], cfl[1])
+        s.set_prim(z["U"], z["turb"] if flow == SST else None)
+        s.step = int(z["step"])
+        s.hist_step = z["hist_step"].copy()
+        s.hist = z["hist"].copy()
+    return s
 
 
 class BatchSolver(Solver):
@@ -306,6 +443,9 @@ class BatchSolver(Solver):
         self.ncase = n.value
         self.case_cell = off.astype(np.int64)
         self.nc = int(off[-1])
+        self.case_params = [dict(mach=float(mach[i]), aoa=float(aoa[i]), reynolds=float(reynolds[i]))
+                            for i in range(k)]
+        self._common(flow, precision, cfl_lusgs, cfl_rk)
 
     def status(self) -> np.ndarray:
         """Per-case status: 0 running, bit 0 HLLC p* check failed, bit 1 slip-wall check failed."""

@@ -18,7 +18,8 @@ t, m, p).  Outputs: the state relative to the free stream, scaled to O(1):
 sin(alpha), (p - 1) / (gamma M^2).
 
 Run ``python -m tmp_amd.warmstart`` for the cold-versus-warm experiment
-(prints progress and one JSON summary line).
+(prints progress and one JSON summary line); ``--checkpoints DIR`` keeps
+each batch of training solves as a checkpoint and reuses it on the next run.
 """
 from __future__ import annotations
 
@@ -26,6 +27,7 @@ import argparse
 import json
 import math
 import multiprocessing
+import os
 import sys
 import time
 from concurrent.futures import ProcessPoolExecutor
@@ -125,17 +127,35 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_dataset(cases, iters, keep_drop=1e-5, batch=64):
+def _same_cases(s, cs) -> bool:
+    return len(s.case_params) == len(cs) and all(
+        p["mach"] == c["mach"] and p["aoa"] == c["aoa"] and p["reynolds"] == c["re"]
+        for p, c in zip(s.case_params, cs))
+
+
+def make_dataset(cases, iters, keep_drop=1e-5, batch=64, checkpoint_dir=None):
     """Converged fields of the cases (laminar, batched solves of `iters`
     iterations); cases that failed or whose residual did not drop by
-    keep_drop are left out."""
+    keep_drop are left out.  With checkpoint_dir, each solved batch is saved
+    there (Solver.save) and a later call loads it instead of solving again."""
     X, Y, kept = [], [], []
     for a in range(0, len(cases), batch):
         cs = cases[a:a + batch]
-        ms = meshes_for(cs)
-        s = batch_solver(ms, cs)
         t0 = time.perf_counter()
-        first, h = iterate_to(s, np.full(len(cs), -1.0), iters, chunk=1000)
+        path = os.path.join(checkpoint_dir, f"train_{a:05d}_{len(cs)}.npz") if checkpoint_dir else None
+        if path and os.path.exists(path):
+            s = usg.load_checkpoint(path)
+            if not _same_cases(s, cs) or s.step != iters:
+                raise ValueError(f"{path} holds other cases or iterations; remove it to solve again")
+            ms = s.meshes
+        else:
+            ms = meshes_for(cs)
+            s = batch_solver(ms, cs)
+            iterate_to(s, np.full(len(cs), -1.0), iters, chunk=1000)
+            if path:
+                os.makedirs(checkpoint_dir, exist_ok=True)
+                s.save(path)
+        h = s.hist
         st = s.status()
         U = s.U
         for i, (m, c) in enumerate(zip(ms, cs)):
@@ -189,11 +209,12 @@ def first_hits(hist: np.ndarray, tol: np.ndarray) -> np.ndarray:
 LEVELS = (1e-4, 1e-6, 1e-8)
 
 
-def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=40000, keep_drop=1e-5):
+def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=40000, keep_drop=1e-5,
+               checkpoint_dir=None):
     t_all = time.perf_counter()
     train_cases = family_cases(n_train, 5005)
     test_cases = family_cases(n_test, 5007)
-    X, Y, kept = make_dataset(train_cases, train_iters, keep_drop)
+    X, Y, kept = make_dataset(train_cases, train_iters, keep_drop, checkpoint_dir=checkpoint_dir)
     log(f"dataset: {len(kept)} fields, {len(X)} points")
     net, losses, t_train = train_net(X, Y, epochs)
     ms = meshes_for(test_cases)
@@ -257,8 +278,9 @@ def main():
     ap.add_argument("--train-iters", type=int, default=20000)
     ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--max-iters", type=int, default=40000)
+    ap.add_argument("--checkpoints", default=None, help="directory of training-solve checkpoints (reused)")
     a = ap.parse_args()
-    r = experiment(a.train, a.test, a.train_iters, a.epochs, a.max_iters)
+    r = experiment(a.train, a.test, a.train_iters, a.epochs, a.max_iters, checkpoint_dir=a.checkpoints)
     print(json.dumps(r), flush=True)
 
 
