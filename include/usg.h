@@ -55,6 +55,7 @@ extern "C" {
 #define USG_F_F1 13      /* SST blending F1 [nc]                                  */
 #define USG_F_DT 14      /* local time step [nc]            getStableTime         */
 #define USG_F_TDW 15     /* turbulence backward increment [nc*2]                   */
+#define USG_F_TCONS 16   /* (rho k, rho omega) [nc*2]       SSTkomegaModel W      */
 
 typedef struct usg_mesh_desc {
   int nn;              /* nodes                                              */
@@ -117,6 +118,15 @@ int usg_sync(usg_ctx *ctx);
 int usg_init_freestream(usg_ctx *ctx);
 /* Warm start from primitive U [nc*4] (and, for SST, (k,omega) [nc*2], may be NULL). */
 int usg_set_prim(usg_ctx *ctx, const double *U, const double *turb);
+/* Restart: the whole pseudo-time state, as usg_get_field reads it --
+ * U [nc*4] (USG_F_PRIM), W [nc*4] (USG_F_CONS; NULL: computed from U), and
+ * for SST turb [nc*2] (USG_F_TURB), tcons [nc*2] (USG_F_TCONS; NULL: from
+ * turb) and mut [nc] (USG_F_MUT).  An iteration carries exactly this state
+ * (W is updated and U derived from it), so a context restarted from a
+ * state read after iteration n continues with the iterates of the original
+ * run.  No counterpart in the reference (it has no restart). */
+int usg_set_state(usg_ctx *ctx, const double *U, const double *W, const double *turb, const double *tcons,
+                  const double *mut);
 
 /* preprocessAdvanceSerial (Euler2D.h:133-140; ViscousFlow2D.h:192-212):
  * boundary states, gradients, spectral radii, limiter, HLLC (+ viscous,

@@ -39,6 +39,9 @@ struct DevMesh {
   const int *cf, *cnb;
   const int *flc, *frc;  // face left cell, right cell or kWallD / kFarD
   const double *fnx, *fny, *fA, *fcx, *fcy;
+  // |x_R - x_L| of interior faces (viscous flows; the r_A distance of
+  // SolveViscousFlux, TopologyHolderStrategy.cpp:143-188)
+  const double *fdist;
   // FP32 copies for the mixed-precision kernels (null in FP64 mode):
   // normal, length, and the offsets fdx[6][nf] = (x_f - x_L, y_f - y_L,
   // x_f - x_R, y_f - y_R, x_R - x_L, y_R - y_L), differenced in FP64 on the
@@ -71,7 +74,8 @@ struct DevState {
                         // viscous spectral 9, SST 10-11), oriented out of the left cell
   double *D;            // [nc] LU-SGS diagonal
   double *tD;           // [2][nc] SST diagonal
-  double *mprc;         // [nc] mu/Pr of the cell (laminar + turbulent) for r_A
+  double *rA;           // [2][nf] LU-SGS r_A of interior faces: [f] into the left cell
+                        // (from the right neighbour), [nf + f] into the right cell
   // SST
   double *tU, *tW, *tUn, *tWn;  // [2][nc]
   double *tgrad;                // [4][nc]: (2q+d)*nc

@@ -327,6 +327,28 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
 }
 
 // ---------------------------------------------------------------------------
+// mu/Pr of a cell, laminar plus turbulent for SST (the r_A viscosity of
+// SolveViscousFlux, TopologyHolderStrategy.cpp:143-188)
+template <bool SST>
+__device__ __forceinline__ double cell_mpr(const DevState &s, const Consts &k, int c) {
+  double mprc = s.mu[c] / k.pr;
+  if (SST) mprc = mprc + s.mut[c] / k.prt;
+  return mprc;
+}
+
+// r_A = omega (|V_j . n| + c_j) (+ max(4/(3 rho_j), gamma/rho_j) (mu/Pr)_i /
+// |x_j - x_i| if viscous) of cell i receiving from neighbour j with state
+// Ua (SolveDF, TopologyHolderStrategy.cpp:217-323)
+template <bool VISC>
+__device__ __forceinline__ double lusgs_ra(const double *Ua, double n0, double n1, double mprc, double dist) {
+  const double Gamma = USG_GAMMA, OMEGAN = 1.5;
+  double Vn = dot2(Ua[2], Ua[3], n0, n1);
+  double cs = sqrt(Gamma * Ua[1] / Ua[0]);
+  double rA = OMEGAN * (fabs(Vn) + cs);
+  if (VISC) rA += smax(4 / (3 * Ua[0]), Gamma / Ua[0]) * (mprc) / dist;
+  return rA;
+}
+
 // k_face: every flux of one face, computed once.  HLLCFluxStrategy::computeFlux
 // (HLLCFluxStrategy.cpp:4-171) with the limited reconstruction of
 // updateLeftRight (HLLCFluxStrategy.h:35-100); the viscous face flux of
@@ -376,6 +398,19 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   // viscous flux 5-8, viscous spectral term 9, SST 10-11
   T *const FF = pick<T>(s.FF, s.FFf) + f;
   for (int j = 0; j < 4; j++) FF[(size_t)j * nf] = Fx[j];
+  if (inner) {
+    // r_A of the sweeps' off-diagonal terms for each side, from the cell
+    // states they would otherwise gather: the left cell receives from the
+    // right (U_R, mu/Pr of L) and the right cell from the left
+    double mprL = 0.0, mprR = 0.0, dist = 0.0;
+    if (F::visc) {
+      mprL = cell_mpr<F::sst>(s, k, L);
+      mprR = cell_mpr<F::sst>(s, k, Rr);
+      dist = m.fdist[f];
+    }
+    s.rA[f] = lusgs_ra<F::visc>(URd, m.fnx[f], m.fny[f], mprL, dist);
+    s.rA[(size_t)nf + f] = lusgs_ra<F::visc>(ULd, m.fnx[f], m.fny[f], mprR, dist);
+  }
   if (inner)
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
   {
@@ -684,11 +719,6 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   double diag = V / dtc + 0.5 * OMEGAN * lamc;
   if (F::visc) diag += lamv;
   s.D[c] = diag;
-  if (F::visc) {
-    double mprc = s.mu[c] / k.pr;
-    if (F::sst) mprc = mprc + s.mut[c] / k.prt;
-    s.mprc[c] = mprc;
-  }
   if (F::sst) {
     double tUc[2] = {s.tU[c], s.tU[nc + c]};
     const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
@@ -756,9 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
   if (!xcd_item(items, sl, c)) return;
   const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc;
-  const double Gamma = USG_GAMMA, OMEGAN = 1.5;
   const double diag = s.D[c];
-  const double mprc = F::visc ? s.mprc[c] : 0.0;
   const double *dsrc = FWD ? s.dW1 : s.dW;
   const double *tsrc = FWD ? s.tdW1 : s.tdW;
   double dFi[4] = {0, 0, 0, 0};
@@ -779,15 +807,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
     }
     cons_flux(W, n0, n1, Fc);
     cons_flux(Wp, n0, n1, Fcp);
-    double Ua[4];
-    load4(s.U, nc, nb, Ua);
-    double Vn = dot2(Ua[2], Ua[3], n0, n1);
-    double cs = sqrt(Gamma * Ua[1] / Ua[0]);
-    double rA = OMEGAN * (fabs(Vn) + cs);
-    if (F::visc) {
-      double cl0 = m.ccx[nb] - m.ccx[c], cl1 = m.ccy[nb] - m.ccy[c];
-      rA += smax(4 / (3 * Ua[0]), Gamma / Ua[0]) * (mprc) / sqrt(dot2(cl0, cl1, cl0, cl1));
-    }
+    const double rA = s.rA[(o.left ? (size_t)0 : (size_t)m.nf) + o.f];
     for (int j = 0; j < 4; j++) {
       double DFc = o.left ? (Fcp[j] - Fc[j]) : (Fc[j] - Fcp[j]);
       double Fv = rA * dW[j];
@@ -1488,8 +1508,19 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
       CK(dalloc(c, &s.gradT, 2 * (size_t)nc));
     CK(dalloc(c, &s.lamv, nc));
     CK(dalloc(c, &s.mu, nc));
-    CK(dalloc(c, &s.mprc, nc));
+    std::vector<double> dist(nf, 0.0);
+    for (int f = 0; f < nf; f++) {
+      const int L = h.flc[f], R = h.frc[f];
+      if (R < 0) continue;
+      double l0 = ccx[R] - ccx[L], l1 = ccy[R] - ccy[L];
+      double d2 = 0;
+      d2 += l0 * l0;
+      d2 += l1 * l1;
+      dist[f] = std::sqrt(d2);
+    }
+    CK(upload(c, (double **)&dm.fdist, dist.data(), nf));
   }
+  CK(dalloc(c, &s.rA, 2 * (size_t)nf));
   if (sst) {
     CK(dalloc(c, &s.tU, 2 * (size_t)nc));
     CK(dalloc(c, &s.tW, 2 * (size_t)nc));
@@ -1590,6 +1621,44 @@ int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
       for (int j = 0; j < 2; j++) t[(size_t)j * nc + i] = turb[2 * (size_t)i + j];
     HIPCHK(hipMemcpyAsync(c->ds.tU, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->have_residual = false;
+  return USG_OK;
+}
+
+static int upload_aos(usg_ctx *c, const double *src, int width, double *dst) {
+  const int nc = c->hm.nc;
+  std::vector<double> soa((size_t)width * nc);
+  for (int i = 0; i < nc; i++)
+    for (int j = 0; j < width; j++) soa[(size_t)j * nc + i] = src[(size_t)width * i + j];
+  HIPCHK(hipMemcpyAsync(dst, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // soa is freed on return
+  return USG_OK;
+}
+
+int usg_set_state(usg_ctx *c, const double *U, const double *W, const double *turb, const double *tcons,
+                  const double *mut) {
+  if (!c || !U) return fail(USG_ERR_ARG, "null argument");
+  const bool sst = c->prm.flow == USG_SST;
+  if (sst && (!turb || !mut)) return fail(USG_ERR_ARG, "SST state needs turb and mut");
+  int rc = clear_case_status(c);
+  if (rc) return rc;
+  if ((rc = upload_aos(c, U, 4, c->ds.U))) return rc;
+  if (W) {
+    if ((rc = upload_aos(c, W, 4, c->ds.W))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
+  }
+  if (sst) {
+    if ((rc = upload_aos(c, turb, 2, c->ds.tU))) return rc;
+    if (tcons) {
+      if ((rc = upload_aos(c, tcons, 2, c->ds.tW))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
+    }
+    if ((rc = upload_aos(c, mut, 1, c->ds.mut))) return rc;
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1787,6 +1856,7 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
     case USG_F_F1: return s.F1f ? soa_to_aos(c, s.F1f, 1, out, count, sst) : soa_to_aos(c, s.F1, 1, out, count, sst);
     case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
     case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
+    case USG_F_TCONS: return soa_to_aos(c, s.tW, 2, out, count, sst);
     default: return fail(USG_ERR_ARG, "unknown field");
   }
 }

@@ -25,11 +25,11 @@ LIB_PATH = os.path.join(HERE, "libusg.so")
 EULER, LAMINAR, SST = 0, 1, 2
 FP64, MIXED = 0, 1
 F_PRIM, F_CONS, F_RES, F_GRAD, F_LAMC, F_PHI, F_DW1, F_DW = range(8)
-F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW = range(8, 16)
+F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW, F_TCONS = range(8, 17)
 K_GRAD, K_FACE, K_CELL, K_SWEEP = range(4)
 
 _FIELD_WIDTH = {F_PRIM: 4, F_CONS: 4, F_RES: 4, F_GRAD: 8, F_LAMC: 1, F_PHI: 4, F_DW1: 4, F_DW: 4,
-                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2}
+                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2, F_TCONS: 2}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -55,7 +55,7 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
-           "usg_write_cell_data", "usg_surface_cp", "usg_write_cp"]
+           "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state"]
 
 _lib = None
 
@@ -74,6 +74,7 @@ def lib():
         L.usg_sync.argtypes = [vp]
         L.usg_init_freestream.argtypes = [vp]
         L.usg_set_prim.argtypes = [vp, _dp, _dp]
+        L.usg_set_state.argtypes = [vp, _dp, _dp, _dp, _dp, _dp]
         L.usg_residual.argtypes = [vp]
         L.usg_lusgs_step.argtypes = [vp, C.c_int, _dp]
         L.usg_rk5_step.argtypes = [vp, C.c_int, _dp]
@@ -264,6 +265,18 @@ class Solver:
             tp = turb.ctypes.data_as(_dp)
         _check(lib().usg_set_prim(self._h, U.ctypes.data_as(_dp), tp))
 
+    def set_state(self, U, W=None, turb=None, tcons=None, mut=None):
+        """usg_set_state: restart from a full state (W, tcons may be None)."""
+        arrs = []
+
+        def p(a, w):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float64).reshape(self.nc, w)
+            arrs.append(a)
+            return a.ctypes.data_as(_dp)
+        _check(lib().usg_set_state(self._h, p(U, 4), p(W, 4), p(turb, 2), p(tcons, 2), p(mut, 1)))
+
     def field(self, fid: int) -> np.ndarray:
         w = _FIELD_WIDTH[fid]
         out = np.zeros(self.nc * w)
@@ -356,16 +369,17 @@ class Solver:
         return curve
 
     def save(self, path: str) -> None:
-        """Checkpoint: meshes, parameters, state (U, and k, omega for SST),
-        iteration count and residual history, in one .npz file."""
+        """Checkpoint: meshes, parameters, the pseudo-time state (U, W, and
+        for SST k, omega, rho k, rho omega, mu_t), iteration count and
+        residual history, in one .npz file."""
         d = {"flow": self.flow, "precision": self.precision, "cfl": np.array(self.cfl), "step": self.step,
              "ncase": self.ncase, "batch": isinstance(self, BatchSolver),
              "mach": np.array([p["mach"] for p in self.case_params]),
              "aoa": np.array([p["aoa"] for p in self.case_params]),
              "reynolds": np.array([p["reynolds"] for p in self.case_params]),
-             "U": self.U, "hist_step": self.hist_step, "hist": self.hist}
+             "U": self.U, "W": self.field(F_CONS), "hist_step": self.hist_step, "hist": self.hist}
         if self.flow == SST:
-            d["turb"] = self.field(F_TURB)
+            d.update(turb=self.field(F_TURB), tcons=self.field(F_TCONS), mut=self.field(F_MUT))
         for i, m in enumerate(self.meshes):
             d.update({f"xy{i}": m.xy, f"fnode{i}": m.fnode, f"flc{i}": m.flc, f"frc{i}": m.frc,
                       f"nc{i}": m.nc, f"name{i}": m.name})
@@ -408,7 +422,9 @@ def load_checkpoint(path: str, precision: int | None = None) -> "Solver":
         else:
             s = Solver(meshes[0], flow, float(z["mach"][0]), float(z["aoa"][0]), float(z["reynolds"][0]), prec,
                        cfl[0], cfl[1])
-        s.set_prim(z["U"], z["turb"] if flow == SST else None)
+        sst = flow == SST
+        s.set_state(z["U"], z["W"], z["turb"] if sst else None, z["tcons"] if sst else None,
+                    z["mut"] if sst else None)
         s.step = int(z["step"])
         s.hist_step = z["hist_step"].copy()
         s.hist = z["hist"].copy()
