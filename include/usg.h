@@ -247,6 +247,38 @@ int usg_write_cell_data(const usg_mesh_desc *mesh, const double *U, const double
  * only. */
 int usg_surface_cp(const usg_mesh_desc *mesh, const double *U, const double *fs, int cap, double *table,
                    double *curve, int *nwall, int *ncurve);
+/* ---- overset connectivity (host only, no GPU needed) ---------------------
+ * UniTioga's tioga::performConnectivity (UniTioga/tioga.cpp:152-212) for
+ * nblocks 2-D blocks in one process, block i being mesh tag i+1: hole maps
+ * from the wall faces (frc == USG_WALL), oriented boxes, the ADT donor
+ * search, donor choice by cell resolution (area), fringes and cell iblanks.
+ * overset_faces[i] (may be NULL, or NULL for a block) flags block i's faces
+ * on its overset boundary (nf ints, nonzero = overset; the reader's
+ * setOverSign faces, whose nodes become mandatory receptors).  Receptors
+ * are nodes: their donor is a cell of another block and the stencil
+ * weights its vertices (computeNodalWeight2D), as dataUpdate interpolates
+ * node values (UniTioga/dataUpdate.cpp:6-133). */
+typedef struct usg_overset usg_overset;
+int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_faces, int nblocks,
+                       usg_overset **out);
+int usg_overset_destroy(usg_overset *ov);
+/* Node iblank[nn] (1 field, 0 hole, -1 receptor) and cell iblank[nc] (0 if a
+ * vertex is a hole, -1 if every vertex is a receptor, else 1;
+ * getCellIblanks, bookkeeping.cpp:654-705) of block `block`; either may be
+ * NULL. */
+int usg_overset_iblank(usg_overset *ov, int block, int *node_iblank, int *cell_iblank);
+/* Receptors whose donor cell lies in block `block` (its INTERPLIST, in the
+ * reference's order): *n receives their count; each array may be NULL.
+ * recv_block/recv_node: the receptor; donor_cell: the donor cell of
+ * `block` (input cell numbering); nw: 3 or 4 weights; donor_node[n*4] and
+ * w[n*4]: the donor cell's vertices and weights (unused slots repeat the
+ * first vertex with weight 0) -- the arrays usg_stencil_apply takes. */
+int usg_overset_receptors(usg_overset *ov, int block, int *n, int *recv_block, int *recv_node, int *donor_cell,
+                          int *nw, int *donor_node, double *w);
+/* Resolutions after connectivity: node_res[nn], cell_res[nc] (cell area;
+ * 1e15 marks cells excluded as donors and mandatory receptor nodes). */
+int usg_overset_resolution(usg_overset *ov, int block, double *node_res, double *cell_res);
+
 /* The reference's "Cp_<step>" text file (Tecplot POINT zone of x and C_p,
  * the first point repeated at the end) of a curve from usg_surface_cp. */
 int usg_write_cp(const char *path, const double *curve, int ncurve);

@@ -245,6 +245,8 @@ void xcd_lists(HostMesh &m) {
 
 }  // namespace
 
+double cell_volume(double xv[4][2], int nv) { return volume(xv, nv); }
+
 std::string cell_vertices(int nn, int nf, const int *fnode, const int *flc, const int *frc, int nc,
                           std::vector<int> &cnode, std::vector<int> &cnv) {
   if (nn <= 0 || nf <= 0 || nc <= 0 || !fnode || !flc || !frc) return "empty mesh";

@@ -64,6 +64,15 @@ std::string cell_vertices(int nn, int nf, const int *fnode, const int *flc, cons
 // angle in radians.
 void freestream_state(double mach, double aoa_deg, double fs[4], double *aoa_rad);
 
+// Area of a triangle or quadrilateral (computeCellVolume2D,
+// mathTools/cellVolume.cpp:21-78).
+double cell_volume(double xv[4][2], int nv);
+
+// Weights f[0..nvert) of point p on the vertices v[2*nvert] of a triangle or
+// quadrilateral (computeNodalWeight2D, mathTools/math.cpp:229-312); 0, or -1
+// for another vertex count, -4 for a degenerate triangle.
+int nodal_weight_2d(const double *v, int nvert, const double *p, double *f);
+
 // Sets the calling thread's error text (usg_last_error) and returns code.
 int set_error(int code, const std::string &msg);
 

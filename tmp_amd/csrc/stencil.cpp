@@ -10,6 +10,7 @@
 #include <string>
 
 #include "../../include/usg.h"
+#include "geometry.h"
 
 namespace {
 
@@ -64,14 +65,12 @@ void quad_order(const double xv[4][2], int map[4]) {
 
 }  // namespace
 
-extern "C" int usg_nodal_weights(int n, const int *nvert, const double *xv, const double *xp, double *w) {
-  if (n < 0 || (n && (!nvert || !xv || !xp || !w))) return -1;
-  for (int r = 0; r < n; r++) {
-    const double *v = xv + 8 * (size_t)r;
-    const double *p = xp + 2 * (size_t)r;
-    double *f = w + 4 * (size_t)r;
+namespace usg {
+
+int nodal_weight_2d(const double *v, int nvert, const double *p, double *f) {
+  {
     f[0] = f[1] = f[2] = f[3] = 0.0;
-    if (nvert[r] == 3) {
+    if (nvert == 3) {
       double a[2][2], b[2];
       for (int i = 0; i < 2; i++) {
         for (int j = 0; j < 2; j++) a[i][j] = v[2 * j + i] - v[4 + i];
@@ -81,7 +80,7 @@ extern "C" int usg_nodal_weights(int n, const int *nvert, const double *xv, cons
       f[0] = b[0];
       f[1] = b[1];
       f[2] = 1. - f[0] - f[1];
-    } else if (nvert[r] == 4) {
+    } else if (nvert == 4) {
       double q[4][2];
       for (int i = 0; i < 4; i++) {
         q[i][0] = v[2 * i];
@@ -128,6 +127,17 @@ extern "C" int usg_nodal_weights(int n, const int *nvert, const double *xv, cons
     } else {
       return -1;  // interpolation is defined for triangles and quadrilaterals
     }
+  }
+  return 0;
+}
+
+}  // namespace usg
+
+extern "C" int usg_nodal_weights(int n, const int *nvert, const double *xv, const double *xp, double *w) {
+  if (n < 0 || (n && (!nvert || !xv || !xp || !w))) return -1;
+  for (int r = 0; r < n; r++) {
+    int rc = usg::nodal_weight_2d(xv + 8 * (size_t)r, nvert[r], xp + 2 * (size_t)r, w + 4 * (size_t)r);
+    if (rc) return rc;
   }
   return 0;
 }

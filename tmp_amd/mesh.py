@@ -600,3 +600,55 @@ def rect(nx: int, ny: int, lx: float = 1.0, ly: float = 1.0, bottom: int = FARFI
     m = faces_from_cells(xy, cells, nv, bc)
     m.name = f"rect_{nx}x{ny}{'_tri' if triangles else ''}"
     return m
+
+
+# ---------------------------------------------------------------------------
+# Overset pair: cylinder O-grid over a Cartesian background
+# ---------------------------------------------------------------------------
+
+def background(nx: int = 400, ny: int = 200, x0: float = -10.0, x1: float = 30.0, y0: float = -10.0,
+               y1: float = 10.0) -> Mesh:
+    """Cartesian background of nx*ny quads on [x0,x1]x[y0,y1], far field all
+    round."""
+    m = rect(nx, ny, x1 - x0, y1 - y0)
+    m.xy = m.xy + np.array([x0, y0])
+    m.name = f"background_{nx}x{ny}"
+    return m
+
+
+def ogrid_radii(nj: int, r0: float, dr1: float, ratio: float, dr_max: float) -> np.ndarray:
+    """nj+1 radii from r0: spacing dr1 growing by ``ratio`` and capped at dr_max."""
+    r = [r0]
+    dr = dr1
+    for _ in range(nj):
+        r.append(r[-1] + dr)
+        dr = min(dr * ratio, dr_max)
+    return np.array(r)
+
+
+def cylinder_ogrid(ni: int = 256, nj: int = 96, d: float = 1.0, dr1: float = 1e-3, ratio: float = 1.08,
+                   dr_max: float = 0.05) -> tuple[Mesh, np.ndarray]:
+    """O-grid of ni*nj quads around a cylinder of diameter d at the origin:
+    wall on the cylinder, overset boundary outside.  The radial spacing
+    starts at dr1 and grows by ``ratio`` up to dr_max (uncapped, 96 layers at
+    1.08 would reach r = 20.7, beyond a background of half-height 10; capped
+    at 0.05 they end at r = 3.4).  Returns the mesh and its overset-face
+    flags (1 on the outer boundary, where the face is far field)."""
+    r = ogrid_radii(nj, 0.5 * d, dr1, ratio, dr_max)
+    th = 2 * np.pi * np.arange(ni) / ni
+    X = r[None, :] * np.cos(th)[:, None]
+    Y = r[None, :] * np.sin(th)[:, None]
+    xy = np.column_stack([X.ravel(), Y.ravel()])
+    nid = np.arange(ni * (nj + 1)).reshape(ni, nj + 1)
+    ip = np.roll(np.arange(ni), -1)
+    q = np.stack([nid[:, :-1], nid[ip, :-1], nid[ip, 1:], nid[:, 1:]], axis=-1).reshape(-1, 4)
+    cells, nv = _assemble(xy, q, np.zeros((0, 3), np.int64))
+    rmid = 0.5 * (r[0] + r[-1])
+
+    def bc(x0, y0, x1, y1):
+        rr = np.hypot(0.5 * (x0 + x1), 0.5 * (y0 + y1))
+        return np.where(rr < rmid, WALL, FARFIELD)
+
+    m = faces_from_cells(xy, cells, nv, bc)
+    m.name = f"cylinder_ogrid_{ni}x{nj}"
+    return m, (m.frc == FARFIELD).astype(np.int32)

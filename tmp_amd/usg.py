@@ -55,7 +55,8 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
-           "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state"]
+           "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state", "usg_overset_create",
+           "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution"]
 
 _lib = None
 
@@ -102,6 +103,11 @@ def lib():
         L.usg_write_cell_data.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_char_p, C.c_int, C.c_int]
         L.usg_surface_cp.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_int, _dp, _dp, _ip, _ip]
         L.usg_write_cp.argtypes = [C.c_char_p, _dp, C.c_int]
+        L.usg_overset_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_ip), C.c_int, C.POINTER(vp)]
+        L.usg_overset_destroy.argtypes = [vp]
+        L.usg_overset_iblank.argtypes = [vp, C.c_int, _ip, _ip]
+        L.usg_overset_receptors.argtypes = [vp, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp]
+        L.usg_overset_resolution.argtypes = [vp, C.c_int, _dp, _dp]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -183,6 +189,47 @@ def write_cp(path: str, curve) -> None:
     """The reference's Cp_<step> text file of a curve from surface_cp."""
     c = np.ascontiguousarray(curve, np.float64).reshape(-1, 4)
     _check(lib().usg_write_cp(os.fsencode(path), c.ctypes.data_as(_dp), c.shape[0]))
+
+
+def overset_connect(meshes, overset_faces=None) -> list[dict]:
+    """UniTioga connectivity of the blocks (usg_overset_*): per block, node
+    and cell iblank, resolutions, and the receptors whose donor cell lies in
+    it (recv_block, recv_node, donor_cell, nw, donor_node[n,4], w[n,4])."""
+    k = len(meshes)
+    descs = (_MeshDesc * k)()
+    keep = []
+    flags = (_ip * k)()
+    for i, m in enumerate(meshes):
+        d, kp = _desc(m)
+        descs[i] = d
+        keep.append(kp)
+        f = None if overset_faces is None or overset_faces[i] is None else \
+            np.ascontiguousarray(overset_faces[i], np.int32)
+        keep.append(f)
+        flags[i] = f.ctypes.data_as(_ip) if f is not None else _ip()
+    L = lib()
+    h = C.c_void_p()
+    _check(L.usg_overset_create(descs, flags, k, C.byref(h)))
+    out = []
+    try:
+        for i, m in enumerate(meshes):
+            nib = np.zeros(m.nn, np.int32)
+            cib = np.zeros(m.nc, np.int32)
+            _check(L.usg_overset_iblank(h, i, nib.ctypes.data_as(_ip), cib.ctypes.data_as(_ip)))
+            nres, cres = np.zeros(m.nn), np.zeros(m.nc)
+            _check(L.usg_overset_resolution(h, i, nres.ctypes.data_as(_dp), cres.ctypes.data_as(_dp)))
+            n = C.c_int()
+            _check(L.usg_overset_receptors(h, i, C.byref(n), None, None, None, None, None, None))
+            r = {"recv_block": np.zeros(n.value, np.int32), "recv_node": np.zeros(n.value, np.int32),
+                 "donor_cell": np.zeros(n.value, np.int32), "nw": np.zeros(n.value, np.int32),
+                 "donor_node": np.zeros((n.value, 4), np.int32), "w": np.zeros((n.value, 4))}
+            _check(L.usg_overset_receptors(h, i, C.byref(n), *(r[q].ctypes.data_as(_ip) for q in
+                                                               ("recv_block", "recv_node", "donor_cell", "nw",
+                                                                "donor_node")), r["w"].ctypes.data_as(_dp)))
+            out.append({"node_iblank": nib, "cell_iblank": cib, "node_res": nres, "cell_res": cres, **r})
+    finally:
+        L.usg_overset_destroy(h)
+    return out
 
 
 ROW, COLUMN = 0, 1
