@@ -42,6 +42,12 @@ struct DevMesh {
   // |x_R - x_L| of interior faces (viscous flows; the r_A distance of
   // SolveViscousFlux, TopologyHolderStrategy.cpp:143-188)
   const double *fdist;
+  // the face normal and length again, per cell slot ([s*nc + c], the slot
+  // order of cf), so that a sweep wave (a contiguous cell range) reads them
+  // coalesced; fslot[f] / fslot[nf + f]: the slot of face f in its left /
+  // right cell's list
+  const double *snx, *sny, *sA;
+  const int *fslot;
   // FP32 copies for the mixed-precision kernels (null in FP64 mode):
   // normal, length, and the offsets fdx[6][nf] = (x_f - x_L, y_f - y_L,
   // x_f - x_R, y_f - y_R, x_R - x_L, y_R - y_L), differenced in FP64 on the
@@ -74,8 +80,8 @@ struct DevState {
                         // viscous spectral 9, SST 10-11), oriented out of the left cell
   double *D;            // [nc] LU-SGS diagonal
   double *tD;           // [2][nc] SST diagonal
-  double *rA;           // [2][nf] LU-SGS r_A of interior faces: [f] into the left cell
-                        // (from the right neighbour), [nf + f] into the right cell
+  double *rA;           // [4][nc] LU-SGS r_A of each interior slot of a cell (from the
+                        // neighbour across it), written by the face kernel
   // SST
   double *tU, *tW, *tUn, *tWn;  // [2][nc]
   double *tgrad;                // [4][nc]: (2q+d)*nc

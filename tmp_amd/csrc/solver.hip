@@ -408,8 +408,8 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
       mprR = cell_mpr<F::sst>(s, k, Rr);
       dist = m.fdist[f];
     }
-    s.rA[f] = lusgs_ra<F::visc>(URd, m.fnx[f], m.fny[f], mprL, dist);
-    s.rA[(size_t)nf + f] = lusgs_ra<F::visc>(ULd, m.fnx[f], m.fny[f], mprR, dist);
+    s.rA[(size_t)m.fslot[f] * nc + L] = lusgs_ra<F::visc>(URd, m.fnx[f], m.fny[f], mprL, dist);
+    s.rA[(size_t)m.fslot[nf + f] * nc + Rr] = lusgs_ra<F::visc>(ULd, m.fnx[f], m.fny[f], mprR, dist);
   }
   if (inner)
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
@@ -797,7 +797,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
     if (o.nb < 0) continue;
     if (FWD ? !(o.nb < c) : !(o.nb > c)) continue;
     const int nb = o.nb;
-    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+    const size_t sc = (size_t)sl * nc + c;
+    const double n0 = m.snx[sc], n1 = m.sny[sc], A = m.sA[sc];
     double W[4], Wp[4], dW[4], Fc[4], Fcp[4];
     for (int j = 0; j < 4; j++) {
       double W0 = s.W[j * nc + nb];
@@ -807,7 +808,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
     }
     cons_flux(W, n0, n1, Fc);
     cons_flux(Wp, n0, n1, Fcp);
-    const double rA = s.rA[(o.left ? (size_t)0 : (size_t)m.nf) + o.f];
+    const double rA = s.rA[sc];
     for (int j = 0; j < 4; j++) {
       double DFc = o.left ? (Fcp[j] - Fc[j]) : (Fc[j] - Fcp[j]);
       double Fv = rA * dW[j];
@@ -1413,13 +1414,18 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
     fcx[f] = h.fc[2 * f];
     fcy[f] = h.fc[2 * f + 1];
   }
-  std::vector<int> cf(4 * (size_t)nc, kNoSlot), cnb(4 * (size_t)nc, kNoSlot);
+  std::vector<int> cf(4 * (size_t)nc, kNoSlot), cnb(4 * (size_t)nc, kNoSlot), fslot(2 * (size_t)nf, 0);
+  std::vector<double> snx(4 * (size_t)nc, 0.0), sny(4 * (size_t)nc, 0.0), sA(4 * (size_t)nc, 0.0);
   for (int i = 0; i < nc; i++)
     for (int s = 0; s < h.cnf[i]; s++) {
       int f = h.cface[4 * i + s];
       bool left = h.flc[f] == i;
       cf[(size_t)s * nc + i] = left ? f : -(f + 1);
       cnb[(size_t)s * nc + i] = left ? h.frc[f] : h.flc[f];
+      fslot[(left ? (size_t)0 : (size_t)nf) + f] = s;
+      snx[(size_t)s * nc + i] = h.fn[2 * f];
+      sny[(size_t)s * nc + i] = h.fn[2 * f + 1];
+      sA[(size_t)s * nc + i] = h.fa[f];
     }
   DevMesh &dm = c->dm;
   dm.nc = nc;
@@ -1430,6 +1436,10 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, (double **)&dm.wdist, h.wdist.data(), nc));
   CK(upload(c, (int **)&dm.cf, cf.data(), cf.size()));
   CK(upload(c, (int **)&dm.cnb, cnb.data(), cnb.size()));
+  CK(upload(c, (int **)&dm.fslot, fslot.data(), fslot.size()));
+  CK(upload(c, (double **)&dm.snx, snx.data(), snx.size()));
+  CK(upload(c, (double **)&dm.sny, sny.data(), sny.size()));
+  CK(upload(c, (double **)&dm.sA, sA.data(), sA.size()));
   CK(upload(c, (int **)&dm.flc, h.flc.data(), nf));
   CK(upload(c, (int **)&dm.frc, h.frc.data(), nf));
   CK(upload(c, (double **)&dm.fnx, fnx.data(), nf));
@@ -1520,7 +1530,7 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
     }
     CK(upload(c, (double **)&dm.fdist, dist.data(), nf));
   }
-  CK(dalloc(c, &s.rA, 2 * (size_t)nf));
+  CK(dalloc(c, &s.rA, 4 * (size_t)nc));
   if (sst) {
     CK(dalloc(c, &s.tU, 2 * (size_t)nc));
     CK(dalloc(c, &s.tW, 2 * (size_t)nc));
