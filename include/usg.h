@@ -278,6 +278,13 @@ int usg_overset_receptors(usg_overset *ov, int block, int *n, int *recv_block, i
 /* Resolutions after connectivity: node_res[nn], cell_res[nc] (cell area;
  * 1e15 marks cells excluded as donors and mandatory receptor nodes). */
 int usg_overset_resolution(usg_overset *ov, int block, double *node_res, double *cell_res);
+/* The connectivity's numerical building blocks, for tests: the partial
+ * median partition of x[n] with companion ix[n] (find_median_wrap,
+ * mathTools/median.cpp:188-232), and the oriented box of n points x[n*2]
+ * (findOBB, UniTioga/utils.cpp:43-292): centre xc[2], half sizes dxc[2],
+ * axes vec[2*2] (row i = axis i). */
+int usg_overset_median(int *ix, double *x, int n);
+int usg_overset_obb(const double *x, int n, double *xc, double *dxc, double *vec);
 
 /* The reference's "Cp_<step>" text file (Tecplot POINT zone of x and C_p,
  * the first point repeated at the end) of a curve from usg_surface_cp. */

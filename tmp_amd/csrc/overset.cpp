@@ -1227,6 +1227,24 @@ int usg_overset_receptors(usg_overset *o, int block, int *n, int *recv_block, in
   return USG_OK;
 }
 
+int usg_overset_median(int *ix, double *x, int n) {
+  if (n < 0 || (n > 0 && (!ix || !x))) return set_error(USG_ERR_ARG, "null argument");
+  find_median_wrap(ix, x, n);
+  return USG_OK;
+}
+
+int usg_overset_obb(const double *x, int n, double *xc, double *dxc, double *vec) {
+  if (n < 1 || !x || !xc || !dxc || !vec) return set_error(USG_ERR_ARG, "null argument");
+  Obb o;
+  find_obb(x, n, o);
+  for (int i = 0; i < 2; i++) {
+    xc[i] = o.xc[i];
+    dxc[i] = o.dxc[i];
+    for (int j = 0; j < 2; j++) vec[2 * i + j] = o.vec[i][j];
+  }
+  return USG_OK;
+}
+
 int usg_overset_resolution(usg_overset *o, int block, double *node_res, double *cell_res) {
   if (!o || block < 0 || block >= (int)o->o.blocks.size()) return set_error(USG_ERR_ARG, "bad block");
   const Block &b = o->o.blocks[block];

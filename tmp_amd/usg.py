@@ -56,7 +56,8 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
            "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state", "usg_overset_create",
-           "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution"]
+           "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution",
+           "usg_overset_median", "usg_overset_obb"]
 
 _lib = None
 
@@ -108,6 +109,8 @@ def lib():
         L.usg_overset_iblank.argtypes = [vp, C.c_int, _ip, _ip]
         L.usg_overset_receptors.argtypes = [vp, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp]
         L.usg_overset_resolution.argtypes = [vp, C.c_int, _dp, _dp]
+        L.usg_overset_median.argtypes = [_ip, _dp, C.c_int]
+        L.usg_overset_obb.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -230,6 +233,23 @@ def overset_connect(meshes, overset_faces=None) -> list[dict]:
     finally:
         L.usg_overset_destroy(h)
     return out
+
+
+def overset_median(x) -> tuple[np.ndarray, np.ndarray]:
+    """find_median_wrap on a copy of x: (partitioned x, companion indices)."""
+    x = np.array(x, np.float64)
+    ix = np.arange(x.size, dtype=np.int32)
+    _check(lib().usg_overset_median(ix.ctypes.data_as(_ip), x.ctypes.data_as(_dp), x.size))
+    return x, ix
+
+
+def overset_obb(pts) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """findOBB of points pts[n, 2]: (centre, half sizes, axes as rows)."""
+    p = np.ascontiguousarray(pts, np.float64).reshape(-1, 2)
+    xc, dxc, vec = np.zeros(2), np.zeros(2), np.zeros((2, 2))
+    _check(lib().usg_overset_obb(p.ctypes.data_as(_dp), p.shape[0], xc.ctypes.data_as(_dp),
+                                 dxc.ctypes.data_as(_dp), vec.ctypes.data_as(_dp)))
+    return xc, dxc, vec
 
 
 ROW, COLUMN = 0, 1
