@@ -442,7 +442,7 @@ struct Block {
     const int icell = elementList[adtElement];
     int nvert, inode[4];
     cell_nodes(icell, nvert, inode);
-    double v[8], frac[4];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0}, frac[4];
     for (int m = 0; m < nvert; m++)
       for (int j = 0; j < D; j++) v[D * m + j] = x[D * (size_t)inode[m] + j];
     const int rc = usg::nodal_weight_2d(v, nvert, xs, frac);
@@ -809,7 +809,7 @@ void search(Block &b) {
     const int nvert = b.nv[n];
     for (int i = 0; i < b.nc[n]; i++) {
       double xmin[3] = {BIGVALUE, BIGVALUE, BIGVALUE}, xmax[3] = {-BIGVALUE, -BIGVALUE, -BIGVALUE};
-      double xd[3], dxc[3];
+      double xd[3] = {0, 0, 0}, dxc[3] = {0, 0, 0};
       for (int m = 0; m < nvert; m++) {
         const int id = D * (b.vconn[n][(size_t)nvert * i + m] - BASE);
         for (int j = 0; j < D; j++) {
