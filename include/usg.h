@@ -254,13 +254,17 @@ int usg_surface_cp(const usg_mesh_desc *mesh, const double *U, const double *fs,
  * search, donor choice by cell resolution (area), fringes and cell iblanks.
  * overset_faces[i] (may be NULL, or NULL for a block) flags block i's faces
  * on its overset boundary (nf ints, nonzero = overset; the reader's
- * setOverSign faces, whose nodes become mandatory receptors).  Receptors
+ * setOverSign faces, whose nodes become mandatory receptors).  mexclude
+ * (cell layers next to the overset boundary that may not donate) and
+ * nfringe (receptor layers around holes) are tioga::setMexclude /
+ * setNfringe: MeshBlock's defaults are 3 and 1, the reference's input file
+ * (test.input, read at Orchestra.cpp:105-110) sets 1 and 2.  Receptors
  * are nodes: their donor is a cell of another block and the stencil
  * weights its vertices (computeNodalWeight2D), as dataUpdate interpolates
  * node values (UniTioga/dataUpdate.cpp:6-133). */
 typedef struct usg_overset usg_overset;
-int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_faces, int nblocks,
-                       usg_overset **out);
+int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_faces, int nblocks, int mexclude,
+                       int nfringe, usg_overset **out);
 int usg_overset_destroy(usg_overset *ov);
 /* Node iblank[nn] (1 field, 0 hole, -1 receptor) and cell iblank[nc] (0 if a
  * vertex is a hole, -1 if every vertex is a receptor, else 1;

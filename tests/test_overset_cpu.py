@@ -198,3 +198,110 @@ def test_oriented_box():
     xc, dxc, vec = usg.overset_obb([[0.0, 0.0], [4.0, 0.0005]])
     # TIOGA_MAX(1e-3, d) * 0.5 expands to (1e-3 > d ? 1e-3 : d * 0.5)
     assert np.array_equal(dxc, [2.0, 1e-3])
+
+
+# The reference's own overset scenario: part1.grd and part2.grd moved by
+# (-0.7, 0.15) (Orchestra::init, Orchestra.cpp:204), wall tag -2, overset
+# tag -2147483647, mexclude 1 and nfringe 2 (test.input, read at
+# Orchestra.cpp:105-110).
+@functools.lru_cache(maxsize=None)
+def ref_pair():
+    import os
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    p1 = mesh.read_grd(os.path.join(g, "part1.grd"), -2, -2147483647)
+    p2 = mesh.read_grd(os.path.join(g, "part2.grd"), -2, -2147483647)
+    p2.xy = p2.xy + np.array([-0.7, 0.15])
+    flags = [(m.frc == mesh.FARFIELD).astype(np.int32) for m in (p1, p2)]
+    return (p1, p2), usg.overset_connect([p1, p2], flags, mexclude=1, nfringe=2)
+
+
+def _inside(poly, pts):
+    """Even-odd point-in-polygon test."""
+    x, y = pts[:, 0], pts[:, 1]
+    inside = np.zeros(len(pts), bool)
+    n = len(poly)
+    for i in range(n):
+        (x0, y0), (x1, y1) = poly[i], poly[(i + 1) % n]
+        cross = ((y0 > y) != (y1 > y)) & (x < (x1 - x0) * (y - y0) / np.where(y1 == y0, 1, y1 - y0) + x0)
+        inside ^= cross
+    return inside
+
+
+def _wall_loop(m, tag=mesh.WALL):
+    """The nodes of the faces tagged `tag` as one closed loop."""
+    w = m.fnode[m.frc == tag]
+    nxt = {int(a): int(b) for a, b in w}
+    start = int(w[0, 0])
+    loop, v = [start], nxt[start]
+    while v != start:
+        loop.append(v)
+        v = nxt[v]
+    assert len(loop) == len(w)
+    return m.xy[loop]
+
+
+def test_reference_pair_stencils_and_fringes():
+    ms, res = ref_pair()
+    for b, r in enumerate(res):
+        assert len(r["recv_node"]) > 0
+        xr = ms[1 - b].xy[r["recv_node"]]
+        w = r["w"]
+        xd = ms[b].xy[r["donor_node"]]
+        assert np.max(np.abs(w.sum(axis=1) - 1)) < 1e-14
+        assert np.max(np.abs(np.einsum("nk,nkd->nd", w, xd) - xr)) < 1e-13
+        assert w.min() > -1e-10 and w.max() < 1 + 1e-10
+        assert set(r["recv_node"].tolist()) == set(np.nonzero(res[1 - b]["node_iblank"] == -1)[0].tolist())
+    for b in range(2):
+        cn, cv = usg.cell_nodes(ms[b])
+        nib = res[b]["node_iblank"]
+        for c in range(ms[b].nc):
+            v = nib[cn[c, :cv[c]]]
+            assert not ((v == 0).any() and (v == 1).any())
+
+
+def test_reference_pair_holes_inside_the_other_wall():
+    (p1, p2), res = ref_pair()
+    nib = res[0]["node_iblank"]
+    inside = _inside(_wall_loop(p2), p1.xy)
+    assert inside.sum() > 10
+    assert np.all(nib[inside] == 0)
+    # part2's wall is not cut by part1: no hole nodes in part2
+    assert np.all(res[1]["node_iblank"] != 0)
+
+
+def test_reference_pair_exclusion_depth():
+    """mexclude = 1: the overset-boundary cell layer and one more layer of
+    part2 may not donate; its overset-boundary nodes are mandatory
+    receptors wherever part1 covers them (the others have no donor at all
+    and stay field nodes, orphans of this two-mesh scenario)."""
+    (p1, p2), res = ref_pair()
+    # cell layers of part2 by face adjacency from its overset boundary
+    nb = p2.neighbours()
+    layer = np.full(p2.nc, -1)
+    front = np.unique(p2.flc[p2.frc == mesh.FARFIELD])
+    layer[front] = 0
+    k = 0
+    while front.size:
+        nxt = np.unique(nb[front][nb[front] >= 0])
+        nxt = nxt[layer[nxt] < 0]
+        k += 1
+        layer[nxt] = k
+        front = nxt
+    cres = res[1]["cell_res"]
+    assert np.all(cres[layer <= 1] == 1e15)
+    assert np.all(cres[layer >= 3] < 1e15)
+    obc_nodes = np.unique(p2.fnode[p2.frc == mesh.FARFIELD])
+    covered = _inside(_wall_loop(p1, mesh.FARFIELD), p2.xy[obc_nodes]) & ~_inside(_wall_loop(p1), p2.xy[obc_nodes])
+    assert covered.sum() > 10 and (~covered).sum() > 10
+    nib = res[1]["node_iblank"][obc_nodes]
+    assert np.all(nib[~covered] == 1)
+    assert np.all(nib != 0)
+    # a covered node stays a field node only when every part1 cell around it
+    # is itself excluded as a donor (within mexclude of part1's own boundary)
+    cn, cv = usg.cell_nodes(p1)
+    xv = p1.xy[np.where(cn >= 0, cn, cn[:, :1])]
+    for v in obc_nodes[covered & (nib == 1)]:
+        w = usg.nodal_weights(cv, xv, np.tile(p2.xy[v], (p1.nc, 1)))
+        holds = np.all((w > -1e-10) & (w < 1 + 1e-10), axis=1)
+        assert holds.any() and np.all(res[0]["cell_res"][holds] == 1e15)
+    assert (nib[covered] == -1).sum() > 20

@@ -44,8 +44,9 @@ constexpr int BIGINT = 2147483647;
 constexpr double TOL = 1.0e-10;
 constexpr int HOLEMAPSIZE = 192;
 constexpr int D = 2;  // dimension
-// tioga defaults (tioga.h:69-100)
-constexpr int kIsym = 3, kMexclude = 3, kNfringe = 1;
+// tioga::isym (tioga.h:69); mexclude and nfringe are parameters
+// (tioga::setMexclude / setNfringe)
+constexpr int kIsym = 3;
 constexpr double kResolutionScale = 1.0;
 
 struct Obb {
@@ -398,6 +399,7 @@ struct Interp {
 
 struct Block {
   int tag = 0, nnodes = 0, ncells = 0;
+  int mexclude = 3, nfringe = 1;  // MeshBlock defaults (MeshBlock.h)
   std::vector<double> x;                  // [2*nnodes]
   std::vector<int> nv, nc;                // per type (ascending vertex count)
   std::vector<std::vector<int>> vconn;    // per type, 1-based node ids
@@ -564,7 +566,7 @@ void preprocess(Block &b) {
         }
     }
   }
-  for (int iex = 0; iex < kMexclude; iex++) {
+  for (int iex = 0; iex < b.mexclude; iex++) {
     int kk = 0;
     for (size_t n = 0; n < b.nc.size(); n++) {
       const int nvert = b.nv[n];
@@ -891,7 +893,7 @@ void process_donors(Block &b, const std::vector<HoleMap> &holemap, std::vector<s
   std::vector<int> mtag(b.nnodes, 0), mtag1(b.nnodes, 0);
   for (int i = 0; i < b.nnodes; i++)
     if (b.iblank[i] == 0) mtag[i] = mtag1[i] = 1;
-  for (int iter = 0; iter < kNfringe; iter++) {
+  for (int iter = 0; iter < b.nfringe; iter++) {
     for (size_t n = 0; n < b.nc.size(); n++) {
       const int nvert = b.nv[n];
       for (int i = 0; i < b.nc[n]; i++) {
@@ -999,7 +1001,7 @@ void reduce_fringes(Block &b) {
   b.iblank_reduced.assign(b.nnodes, 0);
   for (int i = 0; i < b.nnodes; i++) b.iblank_reduced[i] = b.iblank[i] > 0 ? b.iblank[i] : 0;
   std::vector<int> tmp = b.iblank_reduced;
-  for (int iter = 0; iter < kNfringe + 1; iter++) {
+  for (int iter = 0; iter < b.nfringe + 1; iter++) {
     for (size_t n = 0; n < b.nc.size(); n++) {
       const int nvert = b.nv[n];
       for (int i = 0; i < b.nc[n]; i++) {
@@ -1138,9 +1140,10 @@ struct usg_overset {
 
 extern "C" {
 
-int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_faces, int nblocks,
-                       usg_overset **out) {
+int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_faces, int nblocks, int mexclude,
+                       int nfringe, usg_overset **out) {
   if (!meshes || !out || nblocks < 1) return set_error(USG_ERR_ARG, "null argument");
+  if (mexclude < 0 || nfringe < 0) return set_error(USG_ERR_ARG, "negative mexclude or nfringe");
   *out = nullptr;
   std::unique_ptr<usg_overset> h(new usg_overset);
   for (int ib = 0; ib < nblocks; ib++) {
@@ -1148,6 +1151,8 @@ int usg_overset_create(const usg_mesh_desc *meshes, const int *const *overset_fa
     if (!m.xy || !m.fnode || !m.flc || !m.frc) return set_error(USG_ERR_ARG, "null mesh array");
     Block b;
     b.tag = ib + 1;
+    b.mexclude = mexclude;
+    b.nfringe = nfringe;
     b.nnodes = m.nn;
     b.ncells = m.nc;
     b.x.assign(m.xy, m.xy + 2 * (size_t)m.nn);

@@ -104,7 +104,8 @@ def lib():
         L.usg_write_cell_data.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_char_p, C.c_int, C.c_int]
         L.usg_surface_cp.argtypes = [C.POINTER(_MeshDesc), _dp, _dp, C.c_int, _dp, _dp, _ip, _ip]
         L.usg_write_cp.argtypes = [C.c_char_p, _dp, C.c_int]
-        L.usg_overset_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_ip), C.c_int, C.POINTER(vp)]
+        L.usg_overset_create.argtypes = [C.POINTER(_MeshDesc), C.POINTER(_ip), C.c_int, C.c_int, C.c_int,
+                                         C.POINTER(vp)]
         L.usg_overset_destroy.argtypes = [vp]
         L.usg_overset_iblank.argtypes = [vp, C.c_int, _ip, _ip]
         L.usg_overset_receptors.argtypes = [vp, C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp]
@@ -194,10 +195,12 @@ def write_cp(path: str, curve) -> None:
     _check(lib().usg_write_cp(os.fsencode(path), c.ctypes.data_as(_dp), c.shape[0]))
 
 
-def overset_connect(meshes, overset_faces=None) -> list[dict]:
+def overset_connect(meshes, overset_faces=None, mexclude: int = 3, nfringe: int = 1) -> list[dict]:
     """UniTioga connectivity of the blocks (usg_overset_*): per block, node
     and cell iblank, resolutions, and the receptors whose donor cell lies in
-    it (recv_block, recv_node, donor_cell, nw, donor_node[n,4], w[n,4])."""
+    it (recv_block, recv_node, donor_cell, nw, donor_node[n,4], w[n,4]).
+    mexclude / nfringe default to MeshBlock's 3 / 1; the reference's input
+    file uses 1 / 2."""
     k = len(meshes)
     descs = (_MeshDesc * k)()
     keep = []
@@ -212,7 +215,7 @@ def overset_connect(meshes, overset_faces=None) -> list[dict]:
         flags[i] = f.ctypes.data_as(_ip) if f is not None else _ip()
     L = lib()
     h = C.c_void_p()
-    _check(L.usg_overset_create(descs, flags, k, C.byref(h)))
+    _check(L.usg_overset_create(descs, flags, k, int(mexclude), int(nfringe), C.byref(h)))
     out = []
     try:
         for i, m in enumerate(meshes):
