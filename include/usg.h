@@ -209,6 +209,11 @@ int usg_kernel_time(usg_ctx *ctx, double *total_ms, long *launches);
 int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, long src_pts, long dst_pts,
                       int nrecv, const int *recv, const int *nw, const int *donor, const double *w,
                       void *hip_stream);
+/* usg_stencil_apply on host arrays (copied to the GPU and back; synchronous):
+ * the form tioga::dataUpdate (UniTioga/tioga.cpp:326-471) has, updating the
+ * solution arrays registered with registerSolution in place. */
+int usg_stencil_apply_host(const double *src, double *dst, int nvar, int layout, long src_pts, long dst_pts,
+                           int nrecv, const int *recv, const int *nw, const int *donor, const double *w);
 /* Donor weights of n receptor points (host arrays): point r lies in a cell
  * of nvert[r] (3 or 4) vertices xv[r*8 .. r*8+2*nvert[r]) and gets
  * w[r*4 .. r*4+nvert[r]) -- computeNodalWeight2D (mathTools/math.cpp:229-312):

@@ -139,3 +139,34 @@ def test_overset_receptors_interpolate_on_gpu():
         xr = dst_m.xy[r["recv_node"]]
         exact = lin[0] + lin[1] * xr[:, 0] + lin[2] * xr[:, 1]
         assert np.max(np.abs(out[r["recv_node"], 0] - exact)) < 1e-12
+
+
+def test_tioga_adapter_driver():
+    """The C++ Tioga adapter (tmp_amd/host/usg_strategy.h) driven the way
+    Orchestra::init drives tioga (Orchestra.cpp:200-264) on the reference's
+    part1 + part2 pair: after dataUpdate every receptor node holds the other
+    mesh's value (node values i+1 per mesh), every other node its own, and
+    the iblank counts equal the Python front end's."""
+    import os
+    import subprocess
+    from tmp_amd import mesh
+    from .common import GOLDEN
+    exe = os.path.join(os.path.dirname(usg.LIB_PATH), "usg_run")
+    out = subprocess.run([exe, "--overset", os.path.join(GOLDEN, "part1.grd"), os.path.join(GOLDEN, "part2.grd"),
+                          "-2", "-2147483647", "-0.7", "0.15", "1", "2"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rows = [dict(zip(l.split()[2::2], map(int, l.split()[3::2]))) for l in out.stdout.splitlines()
+            if l.startswith("mesh")]
+    assert len(rows) == 2
+    p1 = mesh.read_grd(os.path.join(GOLDEN, "part1.grd"), -2, -2147483647)
+    p2 = mesh.read_grd(os.path.join(GOLDEN, "part2.grd"), -2, -2147483647)
+    p2.xy = p2.xy + np.array([-0.7, 0.15])
+    res = usg.overset_connect([p1, p2], [(m.frc == mesh.FARFIELD).astype(np.int32) for m in (p1, p2)],
+                              mexclude=1, nfringe=2)
+    for r, py in zip(rows, res):
+        nib = py["node_iblank"]
+        assert r["field"] == (nib == 1).sum() and r["hole"] == (nib == 0).sum() and r["receptor"] == (nib == -1).sum()
+        assert r["receptor"] > 0
+        assert r["took_other"] == r["receptor"]
+        assert r["unchanged"] == r["field"] + r["hole"]
+        assert r["cell_receptors"] == (py["cell_iblank"] == -1).sum()

@@ -2111,6 +2111,50 @@ int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, 
   return USG_OK;
 }
 
+int usg_stencil_apply_host(const double *src, double *dst, int nvar, int layout, long src_pts, long dst_pts,
+                           int nrecv, const int *recv, const int *nw, const int *donor, const double *w) {
+  if (!src || !dst || nvar <= 0 || nrecv < 0 || src_pts < 0 || dst_pts < 0 ||
+      (layout != USG_ROW && layout != USG_COLUMN) || (nrecv && (!recv || !nw || !donor || !w)))
+    return fail(USG_ERR_ARG, "bad argument");
+  if (src_pts > 2147483647L || dst_pts > 2147483647L) return fail(USG_ERR_ARG, "point count exceeds int32");
+  if (nrecv == 0) return USG_OK;
+  struct Buffers {
+    std::vector<void *> p;
+    ~Buffers() {
+      for (void *q : p) (void)hipFree(q);
+    }
+    void *get(size_t bytes, hipError_t &e) {
+      void *q = nullptr;
+      e = hipMalloc(&q, std::max<size_t>(bytes, 8));
+      if (e == hipSuccess) p.push_back(q);
+      return q;
+    }
+  } b;
+  hipError_t e = hipSuccess;
+  const size_t ns = (size_t)src_pts * nvar, nd = (size_t)dst_pts * nvar;
+  double *ds = (double *)b.get(ns * sizeof(double), e);
+  if (e == hipSuccess) {
+    double *dd = (double *)b.get(nd * sizeof(double), e);
+    int *dr = e == hipSuccess ? (int *)b.get((size_t)nrecv * sizeof(int), e) : nullptr;
+    int *dn = e == hipSuccess ? (int *)b.get((size_t)nrecv * sizeof(int), e) : nullptr;
+    int *ddn = e == hipSuccess ? (int *)b.get((size_t)nrecv * 4 * sizeof(int), e) : nullptr;
+    double *dw = e == hipSuccess ? (double *)b.get((size_t)nrecv * 4 * sizeof(double), e) : nullptr;
+    if (e != hipSuccess) return fail(USG_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    HIPCHK(hipMemcpy(ds, src, ns * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dd, dst, nd * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dr, recv, (size_t)nrecv * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dn, nw, (size_t)nrecv * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ddn, donor, (size_t)nrecv * 4 * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dw, w, (size_t)nrecv * 4 * sizeof(double), hipMemcpyHostToDevice));
+    int rc = usg_stencil_apply(ds, dd, nvar, layout, src_pts, dst_pts, nrecv, dr, dn, ddn, dw, nullptr);
+    if (rc) return rc;
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(dst, dd, nd * sizeof(double), hipMemcpyDeviceToHost));
+    return USG_OK;
+  }
+  return fail(USG_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+}
+
 int usg_copy_kernel(void *dst, const void *src, long bytes, void *st) {
   if (!dst || !src || bytes < 0 || bytes % 8) return fail(USG_ERR_ARG, "bad argument");
   long n = bytes / 8;

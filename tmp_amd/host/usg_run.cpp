@@ -7,14 +7,79 @@
 //
 //   usg_run <mesh.grd> <wallTag> <farTag> <Euler2D|ViscousFlow2D|SST2D>
 //           <LUSGS|RungeKutta5> <Ma> <Re> <AoA> <maxStep> [minRes] [outputEvery]
+#include <cmath>
 #include <cstdio>
 #include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "usg_strategy.h"
 
+// Overset mode, the sequence of Orchestra::init (Orchestra.cpp:200-264):
+//   usg_run --overset <mesh1.grd> <mesh2.grd> <wallTag> <overTag> <dx> <dy> <mexclude> <nfringe>
+// reads both meshes, moves the second by (dx, dy), runs the connectivity,
+// sets every node value of mesh i to i+1 (4 variables), runs dataUpdate and
+// prints per mesh the iblank counts and how many receptor nodes took the
+// other mesh's value.
+static int overset_main(int argc, char **argv) {
+  if (argc < 10) {
+    std::fprintf(stderr, "usage: %s --overset m1.grd m2.grd wallTag overTag dx dy mexclude nfringe\n", argv[0]);
+    return 2;
+  }
+  usg::UnstructReader rd;
+  rd.setDim(2);
+  rd.setWallSign(std::atoi(argv[4]));
+  rd.setOverSign(std::atoi(argv[5]));
+  rd.addFile(argv[2]);
+  rd.addFile(argv[3]);
+  rd.readAll();
+  rd.make_movement(1, std::atof(argv[6]), std::atof(argv[7]), 0);
+  usg::Tioga tg;
+  int mexclude = std::atoi(argv[8]), nfringe = std::atoi(argv[9]);
+  tg.setMexclude(&mexclude);
+  tg.setNfringe(&nfringe);
+  for (int i = 0; i < rd.nMesh(); i++) tg.registerGridData(i + 1, rd.mesh(i));
+  tg.performConnectivity();
+  const int nequ = 4;
+  std::vector<std::vector<double>> q(rd.nMesh());
+  for (int i = 0; i < rd.nMesh(); i++) {
+    q[i].assign((size_t)nequ * rd.mesh(i).nn, (double)(i + 1));
+    tg.registerSolution(i + 1, q[i].data());
+  }
+  tg.dataUpdate(nequ, 0, 0);
+  for (int i = 0; i < rd.nMesh(); i++) {
+    std::vector<int> ib = tg.iblank(i + 1), ic = tg.iblankCell(i + 1);
+    int field = 0, hole = 0, recv = 0, took = 0, kept = 0, cell_recv = 0;
+    for (int v = 0; v < rd.mesh(i).nn; v++) {
+      field += ib[v] == 1;
+      hole += ib[v] == 0;
+      recv += ib[v] == -1;
+      bool all_other = true, all_own = true;
+      for (int k = 0; k < nequ; k++) {
+        const double x = q[i][(size_t)nequ * v + k];
+        all_other = all_other && std::fabs(x - (double)(2 - i)) < 1e-12;
+        all_own = all_own && x == (double)(i + 1);
+      }
+      took += ib[v] == -1 && all_other;
+      kept += ib[v] != -1 && all_own;
+    }
+    for (int c : ic) cell_recv += c == -1;
+    std::printf("mesh %d nodes %d field %d hole %d receptor %d took_other %d unchanged %d cell_receptors %d\n", i + 1,
+                rd.mesh(i).nn, field, hole, recv, took, kept, cell_recv);
+  }
+  return 0;
+}
+
 int main(int argc, char **argv) {
+  if (argc > 1 && std::string(argv[1]) == "--overset") {
+    try {
+      return overset_main(argc, argv);
+    } catch (const std::exception &e) {
+      std::fprintf(stderr, "error: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc < 10) {
     std::fprintf(stderr,
                  "usage: %s mesh.grd wallTag farTag Euler2D|ViscousFlow2D|SST2D LUSGS|RungeKutta5 Ma Re AoA "

@@ -56,6 +56,14 @@ class UnstructReader {
     meshes_.clear();
     for (const auto &f : files_) meshes_.push_back(read(f));
   }
+  // CobaltReader::make_movement (reader/CobaltReader.h:87-93): translate mesh i
+  void make_movement(int i, double dx, double dy, double /*dz*/) {
+    MeshData &m = meshes_.at(i);
+    for (int k = 0; k < m.nn; k++) {
+      m.xy[2 * (size_t)k] += dx;
+      m.xy[2 * (size_t)k + 1] += dy;
+    }
+  }
 
  private:
   MeshData read(const std::string &filename) {
@@ -224,6 +232,94 @@ class GpuFlowStrategy {
   std::vector<MeshData> meshes_;
   std::vector<std::vector<std::vector<double>>> mirrors_;
   std::vector<std::vector<double *>> ptrs_;
+};
+
+// ---------------------------------------------------------------------------
+// Overset assembly (UniTioga/tioga.h): the calls Orchestra makes
+// (Orchestra.cpp:100-264) -- registerGridData, setMexclude / setNfringe,
+// performConnectivity, registerSolution, dataUpdate -- over usg_overset_*
+// and the GPU stencil apply.  Node solutions are AoS (nvar per node), as
+// registerSolution takes them.
+// ---------------------------------------------------------------------------
+class Tioga {
+ public:
+  ~Tioga() {
+    if (ov_) usg_overset_destroy(ov_);
+  }
+  // mesh tags are 1, 2, ... in registration order; overset faces are those
+  // with the reader's overset tag (here: the far-field faces of the mesh)
+  void registerGridData(int btag, const MeshData &m) {
+    if (btag != (int)meshes_.size() + 1) throw std::runtime_error("tioga: mesh tags must be 1, 2, ...");
+    meshes_.push_back(m);
+    std::vector<int> f(m.nf, 0);
+    for (int i = 0; i < m.nf; i++) f[i] = m.frc[i] == USG_FARFIELD;
+    overset_.push_back(std::move(f));
+    q_.push_back(nullptr);
+  }
+  void setMexclude(int *m) { mexclude_ = *m; }
+  void setNfringe(int *n) { nfringe_ = *n; }
+  void performConnectivity() {
+    std::vector<usg_mesh_desc> d;
+    std::vector<const int *> f;
+    for (size_t i = 0; i < meshes_.size(); i++) {
+      d.push_back(meshes_[i].desc());
+      f.push_back(overset_[i].data());
+    }
+    if (ov_) usg_overset_destroy(ov_);
+    ov_ = nullptr;
+    check(usg_overset_create(d.data(), f.data(), (int)d.size(), mexclude_, nfringe_, &ov_));
+  }
+  // MeshBlock::iblank / iblank_cell of mesh `btag`
+  std::vector<int> iblank(int btag) const { return ib(btag, true); }
+  std::vector<int> iblankCell(int btag) const { return ib(btag, false); }
+  void registerSolution(int btag, double *q) { q_.at(btag - 1) = q; }
+  // tioga::dataUpdate (tioga.cpp:326-471): every receptor node takes the
+  // weighted donor-cell vertex values of the other mesh.  As there, all
+  // donor values are taken before any receptor is written.
+  void dataUpdate(int nvar, int /*interptype*/, int /*at_points*/) {
+    if (!ov_) throw std::runtime_error("tioga: dataUpdate before performConnectivity");
+    std::vector<std::vector<double>> src(meshes_.size());
+    for (size_t b = 0; b < meshes_.size(); b++) {
+      if (!q_[b]) throw std::runtime_error("tioga: Solution data not set, cannot update");
+      src[b].assign(q_[b], q_[b] + (size_t)nvar * meshes_[b].nn);
+    }
+    for (size_t b = 0; b < meshes_.size(); b++) {
+      int n = 0;
+      check(usg_overset_receptors(ov_, (int)b, &n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+      if (n == 0) continue;
+      std::vector<int> rb(n), rn(n), dc(n), nw(n), dn(4 * (size_t)n);
+      std::vector<double> w(4 * (size_t)n);
+      check(usg_overset_receptors(ov_, (int)b, &n, rb.data(), rn.data(), dc.data(), nw.data(), dn.data(), w.data()));
+      // receptors grouped by their mesh
+      for (size_t r = 0; r < meshes_.size(); r++) {
+        std::vector<int> sr, snw, sdn;
+        std::vector<double> sw;
+        for (int k = 0; k < n; k++) {
+          if (rb[k] != (int)r) continue;
+          sr.push_back(rn[k]);
+          snw.push_back(nw[k]);
+          sdn.insert(sdn.end(), &dn[4 * (size_t)k], &dn[4 * (size_t)k + 4]);
+          sw.insert(sw.end(), &w[4 * (size_t)k], &w[4 * (size_t)k + 4]);
+        }
+        if (sr.empty()) continue;
+        check(usg_stencil_apply_host(src[b].data(), q_[r], nvar, USG_ROW, meshes_[b].nn, meshes_[r].nn,
+                                     (int)sr.size(), sr.data(), snw.data(), sdn.data(), sw.data()));
+      }
+    }
+  }
+
+ private:
+  std::vector<int> ib(int btag, bool node) const {
+    const MeshData &m = meshes_.at(btag - 1);
+    std::vector<int> out(node ? m.nn : m.nc);
+    check(usg_overset_iblank(ov_, btag - 1, node ? out.data() : nullptr, node ? nullptr : out.data()));
+    return out;
+  }
+  std::vector<MeshData> meshes_;
+  std::vector<std::vector<int>> overset_;
+  std::vector<double *> q_;
+  int mexclude_ = 3, nfringe_ = 1;
+  usg_overset *ov_ = nullptr;
 };
 
 // ---------------------------------------------------------------------------
