@@ -205,6 +205,9 @@ def main():
     t3 = time.perf_counter()
     ktime_ms, klaunch = solver.kernel_time()
     solver.timing(False)
+    config["monitors"] = {"res_l2_first": float(hist.reshape(a.steps, -1)[0, 0]),
+                          "res_l2_last": float(hist.reshape(a.steps, -1)[-1, 0]),
+                          "sum_dW_rho_sq": float(solver.dw_rho_sq()[0])}
     if kind == "batch":
         st = solver.status()
         config["failed_cases"] = int((st != 0).sum())

@@ -88,6 +88,13 @@ def test_lusgs_trajectory(name, mk, flow, mach, aoa, re, n):
     assert np.all(np.isfinite(gpu.U))
     assert eU <= TOL_TRAJ
     assert eh <= TOL_TRAJ
+    # the reference's printed monitor, sum of dW_rho^2 of the last iteration
+    # (LUSGS_Integrator.cpp:131-153)
+    d = orc.field("dW", 4)[:, 0]
+    mon = float(np.cumsum(d * d)[-1])
+    em = abs(gpu.dw_rho_sq()[0] - mon) / mon
+    print(f"{name}: sum dW_rho^2 {mon:.6e} rel {em:.3e}")
+    assert em <= TOL_TRAJ
     if flow == O.SST:
         et = np.max(relerr(gpu.field(usg.F_TURB), orc.field("tU", 2)))
         print(f"{name}: (k, omega) rel {et:.3e}")

@@ -398,6 +398,15 @@ class Solver:
             return cl, cd
         return float(cl[0]), float(cd[0])
 
+    def dw_rho_sq(self) -> np.ndarray:
+        """The reference's LU-SGS monitor (LUSGSIntegrator::postprocessUpdate,
+        LUSGS_Integrator.cpp:131-153): sum over a case's cells of the last
+        iteration's dW_rho squared, added in cell order; one value per case."""
+        d = self.field(F_DW)[:, 0]
+        c = self.case_cell
+        return np.array([float(np.cumsum(d[c[i]:c[i + 1]] * d[c[i]:c[i + 1]])[-1]) if c[i + 1] > c[i] else 0.0
+                         for i in range(self.ncase)])
+
     def waves(self):
         a, b = C.c_int(), C.c_int()
         _check(lib().usg_sweep_waves(self._h, C.byref(a), C.byref(b)))
