@@ -56,6 +56,7 @@ extern "C" {
 #define USG_F_DT 14      /* local time step [nc]            getStableTime         */
 #define USG_F_TDW 15     /* turbulence backward increment [nc*2]                   */
 #define USG_F_TCONS 16   /* (rho k, rho omega) [nc*2]       SSTkomegaModel W      */
+#define USG_F_W0 17      /* W at the start of the last RK5 step [nc*4]  W_scratch */
 
 typedef struct usg_mesh_desc {
   int nn;              /* nodes                                              */

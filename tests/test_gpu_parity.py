@@ -115,6 +115,11 @@ def test_rk5_trajectory(name, mk, flow):
     e = np.max(relerr(gpu.U, orc.U))
     print(f"{name} rk5: U rel {e:.3e}")
     assert e <= TOL_TRAJ
+    # the reference's RK5 monitor (RungeKuttaStrategy.h:87-105)
+    mon = float(np.cumsum(np.abs(orc.U[:, 0] - orc.field("W0", 4)[:, 0]))[-1])
+    em = abs(gpu.rk_monitor()[0] - mon) / mon
+    print(f"{name} rk5: sum |d rho| {mon:.6e} rel {em:.3e}")
+    assert mon > 0 and em <= TOL_TRAJ
 
 
 def test_deterministic():

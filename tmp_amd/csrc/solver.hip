@@ -1867,6 +1867,7 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
     case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
     case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
     case USG_F_TCONS: return soa_to_aos(c, s.tW, 2, out, count, sst);
+    case USG_F_W0: return soa_to_aos(c, s.W0, 4, out, count, true);
     default: return fail(USG_ERR_ARG, "unknown field");
   }
 }

@@ -25,11 +25,11 @@ LIB_PATH = os.path.join(HERE, "libusg.so")
 EULER, LAMINAR, SST = 0, 1, 2
 FP64, MIXED = 0, 1
 F_PRIM, F_CONS, F_RES, F_GRAD, F_LAMC, F_PHI, F_DW1, F_DW = range(8)
-F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW, F_TCONS = range(8, 17)
+F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW, F_TCONS, F_W0 = range(8, 18)
 K_GRAD, K_FACE, K_CELL, K_SWEEP = range(4)
 
 _FIELD_WIDTH = {F_PRIM: 4, F_CONS: 4, F_RES: 4, F_GRAD: 8, F_LAMC: 1, F_PHI: 4, F_DW1: 4, F_DW: 4,
-                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2, F_TCONS: 2}
+                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2, F_TCONS: 2, F_W0: 4}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -405,6 +405,15 @@ class Solver:
         d = self.field(F_DW)[:, 0]
         c = self.case_cell
         return np.array([float(np.cumsum(d[c[i]:c[i + 1]] * d[c[i]:c[i + 1]])[-1]) if c[i + 1] > c[i] else 0.0
+                         for i in range(self.ncase)])
+
+    def rk_monitor(self) -> np.ndarray:
+        """The reference's RK5 monitor (RungeKuttaStrategy::converged,
+        RungeKuttaStrategy.h:87-105): sum over a case's cells of
+        |rho - rho at the start of the last step|, added in cell order."""
+        d = np.abs(self.U[:, 0] - self.field(F_W0)[:, 0])
+        c = self.case_cell
+        return np.array([float(np.cumsum(d[c[i]:c[i + 1]])[-1]) if c[i + 1] > c[i] else 0.0
                          for i in range(self.ncase)])
 
     def waves(self):
