@@ -32,6 +32,33 @@ def airfoil_small():
     return mesh.prepare(mesh.airfoil_hybrid(ni=64, dy1=1e-4, rfar=20.0))[0]
 
 
+def two_cells() -> mesh.Mesh:
+    """Quad (0,1,2,3) on the unit square and triangle (1,4,2) to its right.
+    Faces in file order: (0,1) far field, (1,2) interior, (2,3) far field,
+    (3,0) wall, (1,4) far field, (4,2) wall."""
+    xy = np.array([[0, 0], [1, 0], [1, 1], [0, 1], [2, 0.5]], np.float64)
+    fnode = np.array([[0, 1], [1, 2], [2, 3], [3, 0], [1, 4], [4, 2]], np.int32)
+    flc = np.array([0, 0, 0, 0, 1, 1], np.int32)
+    frc = np.array([mesh.FARFIELD, 1, mesh.FARFIELD, mesh.WALL, mesh.FARFIELD, mesh.WALL], np.int32)
+    return mesh.Mesh(xy, fnode, flc, frc, 2, "two_cells")
+
+
+def one_triangle() -> mesh.Mesh:
+    """A single triangle: wall face (0,1), far-field faces (1,2), (2,0)."""
+    xy = np.array([[0.0, 0.0], [1.0, 0.0], [0.2, 0.9]])
+    fnode = np.array([[0, 1], [1, 2], [2, 0]], np.int32)
+    frc = np.array([mesh.WALL, mesh.FARFIELD, mesh.FARFIELD], np.int32)
+    return mesh.Mesh(xy, fnode, np.zeros(3, np.int32), frc, 1, "one_triangle")
+
+
+def one_quad() -> mesh.Mesh:
+    """A single skewed quadrilateral: wall face (0,1), three far-field faces."""
+    xy = np.array([[0.0, 0.0], [1.0, 0.1], [1.1, 1.0], [-0.1, 0.8]])
+    fnode = np.array([[0, 1], [1, 2], [2, 3], [3, 0]], np.int32)
+    frc = np.array([mesh.WALL, mesh.FARFIELD, mesh.FARFIELD, mesh.FARFIELD], np.int32)
+    return mesh.Mesh(xy, fnode, np.zeros(4, np.int32), frc, 1, "one_quad")
+
+
 def perturbed(m, cc, flow_fs, amp=0.05, seed=0):
     """Smooth perturbation of the free stream (so gradients and the limiter
     are exercised), as AoS U[nc,4]."""

@@ -10,7 +10,7 @@ import pytest
 
 from tmp_amd import usg
 
-from .common import part2
+from .common import one_quad, one_triangle, part2, two_cells
 
 INCLUDE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
 
@@ -82,3 +82,29 @@ def test_batch_rejects_mixed_flow_types():
     assert usg.lib().usg_batch_create(descs, params, 2, C.byref(h)) != 0 and not h.value
     assert b"share flow type" in usg.lib().usg_last_error()
     assert usg.lib().usg_batch_create(descs, params, 0, C.byref(h)) != 0
+
+
+def test_empty_mesh_rejected():
+    z = np.zeros(0, np.int32)
+    empty = usg._mesh.Mesh(np.zeros((0, 2)), np.zeros((0, 2), np.int32), z, z, 0, "empty")
+    with pytest.raises(usg.UsgError, match="empty mesh"):
+        usg.mesh_geometry(empty)
+
+
+def test_tiny_mesh_geometry():
+    """One- and two-cell meshes: shoelace volume, vertex-mean centre, the
+    projected wall distance |(x_c - x_f) . n_f| and one wave per cell
+    colour (UnstructBlock.h:265-298, cellVolume.cpp:6-78)."""
+    for m, nw in ((one_triangle(), 1), (one_quad(), 1), (two_cells(), 2)):
+        g = usg.mesh_geometry(m)
+        assert (g["nfwd"], g["nbwd"]) == (nw, nw)
+        cnode, cnv = usg.cell_nodes(m)
+        for c in range(m.nc):
+            v = m.xy[cnode[c, :cnv[c]]]
+            x, y = v[:, 0], v[:, 1]
+            area = 0.5 * abs(np.dot(x, np.roll(y, -1)) - np.dot(y, np.roll(x, -1)))
+            assert g["vol"][c] == pytest.approx(area, rel=1e-15)
+            assert np.allclose(g["cc"][c], v.mean(axis=0), rtol=0, atol=1e-15)
+            walls = [f for f in range(m.nf) if m.frc[f] == -2]
+            d = min(abs(np.dot(g["cc"][c] - g["fc"][f], g["fn"][f])) for f in walls)
+            assert g["wdist"][c] == pytest.approx(d, rel=1e-14)

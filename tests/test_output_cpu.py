@@ -16,18 +16,9 @@ import pytest
 from tmp_amd import mesh, usg
 from oracle import oracle as O
 
+from .common import two_cells
+
 HERE = os.path.dirname(os.path.abspath(__file__))
-
-
-def two_cells() -> mesh.Mesh:
-    """Quad (0,1,2,3) on the unit square and triangle (1,4,2) to its right.
-    Faces in file order: (0,1) far field, (1,2) interior, (2,3) far field,
-    (3,0) wall, (1,4) far field, (4,2) wall."""
-    xy = np.array([[0, 0], [1, 0], [1, 1], [0, 1], [2, 0.5]], np.float64)
-    fnode = np.array([[0, 1], [1, 2], [2, 3], [3, 0], [1, 4], [4, 2]], np.int32)
-    flc = np.array([0, 0, 0, 0, 1, 1], np.int32)
-    frc = np.array([mesh.FARFIELD, 1, mesh.FARFIELD, mesh.WALL, mesh.FARFIELD, mesh.WALL], np.int32)
-    return mesh.Mesh(xy, fnode, flc, frc, 2, "two_cells")
 
 
 # rho, p, u, v per cell; free stream (rho, p, u, v) = (1, 1, 2, 0)
