@@ -84,7 +84,10 @@ typedef struct usg_ctx usg_ctx;
  * HIP stream.  Replaces WrapSolver::setTopology/setParams/init
  * (SolverWrapper.h:22-68): builds geometry (UnstructBlock.h:572-586), the
  * free stream (TopologyHolderStrategy.cpp:325-351) and the LU-SGS wave
- * schedule, uploads SoA arrays.  Contexts are independent. */
+ * schedule, uploads SoA arrays.  Contexts are independent.  A mesh (or a
+ * batch's union) may hold at most 268435455 cells and 357913941 faces (the
+ * kernels' 32-bit array offsets); a larger one returns USG_ERR_ARG before
+ * any array is read. */
 int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out);
 /* Batched many-case context: ncases independent meshes with their own free
  * streams (mach, aoa, reynolds may differ; flow type, precision and CFL

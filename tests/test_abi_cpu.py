@@ -108,3 +108,19 @@ def test_tiny_mesh_geometry():
             walls = [f for f in range(m.nf) if m.frc[f] == -2]
             d = min(abs(np.dot(g["cc"][c] - g["fc"][f], g["fn"][f])) for f in walls)
             assert g["wdist"][c] == pytest.approx(d, rel=1e-14)
+
+
+def test_oversized_mesh_rejected_before_arrays_are_read():
+    """8 * nc must fit the kernels' 32-bit offsets: a descriptor claiming
+    more cells is refused before its (here far too short) arrays are read."""
+    m = part2()
+    d, keep = usg._desc(m)
+    d.nc = 268435456
+    h = C.c_void_p()
+    assert usg.lib().usg_ctx_create(C.byref(d), C.byref(_params()), C.byref(h)) == -1   # USG_ERR_ARG
+    assert b"32-bit array offsets" in usg.lib().usg_last_error() and not h.value
+    d.nc = 150000000
+    descs = (usg._MeshDesc * 2)(d, d)
+    params = (usg._Params * 2)(_params(), _params())
+    assert usg.lib().usg_batch_create(descs, params, 2, C.byref(h)) == -1
+    assert b"32-bit array offsets" in usg.lib().usg_last_error()

@@ -1334,11 +1334,20 @@ static int check_params(const usg_params *params) {
 
 static int ctx_setup(usg_ctx *c, usg_ctx **out);
 
+// The kernels form structure-of-arrays offsets var*nc + cell (var < 8) and
+// q*nf + face (q < 6, the FP32 face offsets) in 32-bit arithmetic.
+static int check_size(long long nc, long long nf) {
+  if (nc * 8 > 2147483647LL || nf * 6 > 2147483647LL)
+    return fail(USG_ERR_ARG, "mesh exceeds 32-bit array offsets (at most 268435455 cells and 357913941 faces)");
+  return USG_OK;
+}
+
 int usg_ctx_create(const usg_mesh_desc *mesh, const usg_params *params, usg_ctx **out) {
   if (!mesh || !params || !out) return fail(USG_ERR_ARG, "null argument");
   *out = nullptr;
   int rc = check_params(params);
   if (rc) return rc;
+  if ((rc = check_size(mesh->nc, mesh->nf))) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
   usg_ctx *c = new usg_ctx();
@@ -1362,16 +1371,19 @@ int usg_batch_create(const usg_mesh_desc *meshes, const usg_params *params, int 
         params[i].cfl_lusgs != params[0].cfl_lusgs || params[i].cfl_rk != params[0].cfl_rk)
       return fail(USG_ERR_ARG, "cases of a batch share flow type, precision and CFL numbers");
   }
+  long long total_nc = 0, total_nf = 0;
+  for (int i = 0; i < ncases; i++) {
+    total_nc += meshes[i].nc;
+    total_nf += meshes[i].nf;
+  }
+  if (int rc = check_size(total_nc, total_nf)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(USG_ERR_NODEVICE, "no GPU visible");
   std::vector<HostMesh> cases(ncases);
-  long total = 0;
   for (int i = 0; i < ncases; i++) {
     const usg_mesh_desc &d = meshes[i];
     std::string e = build_mesh(cases[i], d.nn, d.xy, d.nf, d.fnode, d.flc, d.frc, d.nc);
     if (!e.empty()) return fail(USG_ERR_MESH, "case " + std::to_string(i) + ": " + e);
-    total += (long)d.nc + d.nf;
-    if (total > 2000000000L) return fail(USG_ERR_ARG, "batch exceeds 32-bit cell/face indexing");
   }
   usg_ctx *c = new usg_ctx();
   c->prm = params[0];
