@@ -21,6 +21,7 @@ oracle and the GPU solver are always run on the same renumbered mesh.
 from __future__ import annotations
 
 import dataclasses
+import heapq
 import math
 import os
 
@@ -233,37 +234,119 @@ def cell_centres(m: Mesh) -> np.ndarray:
     return acc / cnt[:, None]
 
 
-def colour_hilbert_order(m: Mesh, parts: int = 8) -> tuple[np.ndarray, int]:
-    """Permutation ``perm`` (new -> old cell) grouping cells by greedy colour.
+def _dsatur_ball(nbl: list, col: list, ball: list, ncolours: int) -> bool:
+    """Colours the (uncoloured) cells of ``ball`` with at most ``ncolours``
+    colours by DSatur (Brelaz 1979), the colours outside the ball fixed: the
+    next cell is the one with the most distinct colours around it (ties: more
+    neighbours), and it takes the smallest free colour.  False, with the ball
+    partly coloured, when some cell has no free colour."""
+    inb = set(ball)
+    sat = {}
+    for v in ball:
+        m = 0
+        for o in nbl[v]:
+            if o not in inb and col[o] >= 0:
+                m |= 1 << col[o]
+        sat[v] = m
+    nsat = {v: bin(sat[v]).count("1") for v in ball}
+    heap = [(-nsat[v], -len(nbl[v]), v) for v in ball]
+    heapq.heapify(heap)
+    while heap:
+        s, _, v = heapq.heappop(heap)
+        if col[v] >= 0 or -s != nsat[v]:
+            continue
+        k = 0
+        while k < ncolours and sat[v] & (1 << k):
+            k += 1
+        if k == ncolours:
+            return False
+        col[v] = k
+        for o in nbl[v]:
+            if o in inb and col[o] < 0 and not sat[o] & (1 << k):
+                sat[o] |= 1 << k
+                nsat[o] += 1
+                heapq.heappush(heap, (-nsat[o], -len(nbl[o]), o))
+    return True
 
-    Cells are visited in Hilbert order of their centres and given the smallest
-    colour not used by an already coloured neighbour; the new numbering sorts
-    by (colour, spatial part, Hilbert key), the parts being ``parts`` equal
-    slices of the Hilbert curve.  No two cells of one colour are neighbours, so a
-    forward LU-SGS sweep over this numbering can process each colour block as
-    one parallel wave (depth = number of colours), while the Hilbert order
-    inside each block keeps neighbour gathers local.
-    Returns ``(perm, ncolours)``.
+
+def _three_colours(nbl: list, visit: list, max_radius: int = 8) -> list:
+    """Proper colouring with colours 0-2 for all but a few cells.
+
+    Greedy over ``visit`` (the smallest colour no coloured neighbour uses)
+    leaves the cells whose neighbours already hold all three colours.  Each
+    is repaired by recolouring the ball of cells within 1, 2, ...
+    ``max_radius`` faces of it with DSatur, everything outside held fixed.
+    A 3-colouring of a quadrilateral grid can hold defects no local change
+    removes; the cells still left are repaired the same way with a fourth
+    colour allowed, which then holds only a few cells.
+    """
+    col = [-1] * len(nbl)
+    left = []
+    for v in visit:
+        used = 0
+        for o in nbl[v]:
+            if col[o] >= 0:
+                used |= 1 << col[o]
+        k = 0
+        while k < 3 and used & (1 << k):
+            k += 1
+        if k == 3:
+            left.append(v)
+        else:
+            col[v] = k
+    def repair(v, ncolours):
+        for r in range(1, max_radius + 1):
+            ball, seen, front = [v], {v}, [v]
+            for _ in range(r):
+                nxt = []
+                for x in front:
+                    for o in nbl[x]:
+                        if o not in seen:
+                            seen.add(o)
+                            nxt.append(o)
+                ball += nxt
+                front = nxt
+            saved = [col[x] for x in ball]
+            for x in ball:
+                col[x] = -1
+            if _dsatur_ball(nbl, col, ball, ncolours):
+                return True
+            for x, c in zip(ball, saved):
+                col[x] = c
+        return False
+
+    still = [v for v in left if col[v] < 0 and not repair(v, 3)]
+    # the rest: the same repair with a fourth colour, which DSatur gives only
+    # where colours 0-2 are all taken
+    for v in still:
+        if col[v] < 0 and not repair(v, 4):
+            used = {col[o] for o in nbl[v]}
+            col[v] = min(k for k in range(6) if k not in used)
+    return col
+
+
+def colour_hilbert_order(m: Mesh, parts: int = 8) -> tuple[np.ndarray, int]:
+    """Permutation ``perm`` (new -> old cell) grouping cells by colour.
+
+    Cells are coloured in Hilbert order of their centres with three colours
+    (``_three_colours``: greedy, local repair, a fourth colour only for the
+    few cells no local repair fixes).  The new numbering sorts by (colour,
+    spatial part, Hilbert key), the parts being ``parts`` equal slices of
+    the Hilbert curve.  No two cells of one colour are neighbours, so a
+    forward LU-SGS sweep over this numbering can process each colour block
+    as one parallel wave (depth = number of colours), while the Hilbert
+    order inside each block keeps neighbour gathers local.  Few colours mean
+    few waves: a wave's launch costs its threads' dependent chain of loads
+    and divisions however few cells it holds.  (A plain greedy colouring
+    needs five, the last two holding ~1% of the cells; a DSatur colouring of
+    the whole mesh gets three, but its near two-colour, red-black ordering
+    slows the LU-SGS convergence.)  Returns ``(perm, ncolours)``.
     """
     c = cell_centres(m)
     key = _hilbert_keys(c[:, 0], c[:, 1])
     visit = np.argsort(key, kind="stable")
-    nb = m.neighbours()
-    colour = np.full(m.nc, -1, dtype=np.int64)
-    nbl = nb.tolist()
-    col = colour.tolist()
-    for cell in visit.tolist():
-        used = 0
-        for o in nbl[cell]:
-            if o >= 0:
-                co = col[o]
-                if co >= 0:
-                    used |= 1 << co
-        k = 0
-        while used & (1 << k):
-            k += 1
-        col[cell] = k
-    colour = np.array(col, dtype=np.int64)
+    nbl = [[o for o in row if o >= 0] for row in m.neighbours().tolist()]
+    colour = np.array(_three_colours(nbl, visit.tolist()), dtype=np.int64)
     # 8 equal spatial parts along the Hilbert curve (one per XCD of the GPU):
     # inside a colour block the cells are grouped by part, then Hilbert order
     rank = np.empty(m.nc, dtype=np.int64)
