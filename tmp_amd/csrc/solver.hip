@@ -780,11 +780,39 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
 // a dependency level).  The backward wave also applies
 // UpdatePrimitiveVariable (LUSGS_Integrator.cpp:242-255) into the second
 // state buffer, and the SST update (SSTKWStrategy.h:194-223).
+// Blocks from nb.nsweep on run residual-norm work instead (NormBlocks).
 // ---------------------------------------------------------------------------
+__device__ void norm_partial_block(const DevMesh &m, const double *R, const int *b0, const int *b1,
+                                   double *partial, int b);
+__device__ void norm_ring_block(const double *partial, const int *boff, double *ring, int cap, int *ctr, int cs,
+                                int ncase);
+
+// Residual-norm blocks appended to a sweep launch of a single case with
+// kBlock-thread blocks: launch blocks [nsweep, nsweep + n) run the norm's
+// per-block partial sums (kind 1: n = one per norm block) or its ring sum
+// (kind 2: n = 1), the same code on the same cells as k_norm_partial and
+// k_norm_ring, which they replace.  R is not written by the sweeps, and the
+// ring sum runs in the launch after the partial sums.
+struct NormBlocks {
+  int kind = 0, nsweep = 0, cap = 0;
+  const double *R = nullptr;
+  const int *b0 = nullptr, *b1 = nullptr, *boff = nullptr;
+  double *partial = nullptr, *ring = nullptr;
+  int *ctr = nullptr;
+};
+
 template <int FLOW, bool FWD, typename T, bool B>
 __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k_,
-                                                 const int *__restrict__ items, const XSlice sl) {
+                                                 const int *__restrict__ items, const XSlice sl,
+                                                 const NormBlocks nb) {
   using F = Flow<FLOW>;
+  if (nb.kind && (int)blockIdx.x >= nb.nsweep) {
+    if (nb.kind == 1)
+      norm_partial_block(m, nb.R, nb.b0, nb.b1, nb.partial, (int)blockIdx.x - nb.nsweep);
+    else
+      norm_ring_block(nb.partial, nb.boff, nb.ring, nb.cap, nb.ctr, 0, 1);
+    return;
+  }
   int c;
   if (!xcd_item(items, sl, c)) return;
   const Consts k = case_consts<B>(m, k_, c);
@@ -909,12 +937,15 @@ __global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const De
 // Density-residual norm ||R_rho / V||_2 per case.  Block b sums the squares
 // of cells [b0[b], b1[b]) (never two cases in one block); one block per case
 // then sums its blocks' partials.
-__global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const double *R, const int *b0,
-                                                        const int *b1, double *partial) {
+// The block bodies take the block's index explicitly, so that the same code
+// also runs as blocks appended to a sweep launch (NormBlocks); they need
+// kBlock threads.
+__device__ void norm_partial_block(const DevMesh &m, const double *R, const int *b0, const int *b1,
+                                   double *partial, int b) {
   __shared__ double sh[kBlock];
-  const int c = b0[blockIdx.x] + threadIdx.x;
+  const int c = b0[b] + threadIdx.x;
   double v = 0;
-  if (c < b1[blockIdx.x]) {
+  if (c < b1[b]) {
     double r = R[c] / m.vol[c];
     v = r * r;
   }
@@ -924,11 +955,15 @@ __global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const 
     if (threadIdx.x < w) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+  if (threadIdx.x == 0) partial[b] = sh[0];
 }
-__device__ __forceinline__ double case_sum(const double *partial, const int *boff) {
+__global__ __launch_bounds__(kBlock) void k_norm_partial(const DevMesh m, const double *R, const int *b0,
+                                                        const int *b1, double *partial) {
+  norm_partial_block(m, R, b0, b1, partial, blockIdx.x);
+}
+__device__ __forceinline__ double case_sum(const double *partial, const int *boff, int cs) {
   __shared__ double sh[kBlock];
-  const int i0 = boff[blockIdx.x], n = boff[blockIdx.x + 1] - i0;
+  const int i0 = boff[cs], n = boff[cs + 1] - i0;
   double v = 0;
   for (int i = threadIdx.x; i < n; i += kBlock) v = v + partial[i0 + i];
   sh[threadIdx.x] = v;
@@ -941,20 +976,24 @@ __device__ __forceinline__ double case_sum(const double *partial, const int *bof
 }
 // one block per case: out[case] = sqrt(sum)
 __global__ __launch_bounds__(kBlock) void k_norm_final(const double *partial, const int *boff, double *out) {
-  double v = case_sum(partial, boff);
+  double v = case_sum(partial, boff, blockIdx.x);
   if (threadIdx.x == 0) out[blockIdx.x] = sqrt(v);
 }
 // one block per case: ring[(ctr % cap) * ncase + case] = sqrt(sum); the
 // counter is advanced by this kernel when there is one case, else by
 // k_add_int launched after it (no block may see another's increment)
-__global__ __launch_bounds__(kBlock) void k_norm_ring(const double *partial, const int *boff, double *ring,
-                                                      int cap, int *ctr) {
-  double v = case_sum(partial, boff);
+__device__ void norm_ring_block(const double *partial, const int *boff, double *ring, int cap, int *ctr, int cs,
+                                int ncase) {
+  double v = case_sum(partial, boff, cs);
   if (threadIdx.x == 0) {
     int i = *ctr;
-    ring[(size_t)(i % cap) * gridDim.x + blockIdx.x] = sqrt(v);
-    if (gridDim.x == 1) *ctr = i + 1;
+    ring[(size_t)(i % cap) * ncase + cs] = sqrt(v);
+    if (ncase == 1) *ctr = i + 1;
   }
+}
+__global__ __launch_bounds__(kBlock) void k_norm_ring(const double *partial, const int *boff, double *ring,
+                                                      int cap, int *ctr) {
+  norm_ring_block(partial, boff, ring, cap, ctr, blockIdx.x, gridDim.x);
 }
 __global__ void k_add_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
@@ -1222,31 +1261,48 @@ static void launch_residual_any(usg_ctx *c) {
   launch_residual_events(c, e.first, e.second);
 }
 
+// With fuse_norm, the first two forward launches also carry the residual
+// norm of this iteration into the ring (NormBlocks).
 template <int FLOW, typename T, bool B>
-static void launch_sweeps(usg_ctx *c) {
+static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
   const HostMesh &h = c->hm;
   const int bs = c->blk[USG_K_SWEEP];
   // wave 0 of each sweep is fused into k_cell / the forward waves (wflag)
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
   for (int w = 1; w < nw; w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
-                       c->k, c->d_fwd, sl);
+    NormBlocks nb;
+    int extra = 0;
+    if (fuse_norm && w <= 2) {
+      nb.kind = w;
+      nb.nsweep = xgrid(sl, bs);
+      nb.cap = kRing;
+      nb.R = c->ds.R;
+      nb.b0 = c->d_nb0;
+      nb.b1 = c->d_nb1;
+      nb.boff = c->d_nboff;
+      nb.partial = c->d_partial;
+      nb.ring = c->d_ring;
+      nb.ctr = c->d_ctr;
+      extra = w == 1 ? c->nnb : 1;
+    }
+    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl, bs) + extra), dim3(bs), 0, c->stream, c->dm,
+                       c->ds, c->k, c->d_fwd, sl, nb);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
   for (int w = 1; w < nb; w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
     hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
-                       c->k, c->d_bwd, sl);
+                       c->k, c->d_bwd, sl, NormBlocks{});
   }
 }
 
 template <typename T, bool B>
-static void launch_sweeps_t(usg_ctx *c) {
+static void launch_sweeps_t(usg_ctx *c, bool fuse_norm) {
   switch (c->prm.flow) {
-    case USG_EULER: launch_sweeps<USG_EULER, T, B>(c); break;
-    case USG_LAMINAR: launch_sweeps<USG_LAMINAR, T, B>(c); break;
-    default: launch_sweeps<USG_SST, T, B>(c); break;
+    case USG_EULER: launch_sweeps<USG_EULER, T, B>(c, fuse_norm); break;
+    case USG_LAMINAR: launch_sweeps<USG_LAMINAR, T, B>(c, fuse_norm); break;
+    default: launch_sweeps<USG_SST, T, B>(c, fuse_norm); break;
   }
 }
 
@@ -1699,12 +1755,12 @@ int usg_residual(usg_ctx *c) {
   return check_err(c);
 }
 
-static void launch_sweeps_any(usg_ctx *c) {
+static void launch_sweeps_any(usg_ctx *c, bool fuse_norm) {
   const bool mixed = c->prm.precision == USG_MIXED, batch = c->ncase > 1;
   if (mixed)
-    batch ? launch_sweeps_t<float, true>(c) : launch_sweeps_t<float, false>(c);
+    batch ? launch_sweeps_t<float, true>(c, fuse_norm) : launch_sweeps_t<float, false>(c, fuse_norm);
   else
-    batch ? launch_sweeps_t<double, true>(c) : launch_sweeps_t<double, false>(c);
+    batch ? launch_sweeps_t<double, true>(c, fuse_norm) : launch_sweeps_t<double, false>(c, fuse_norm);
 }
 
 static void launch_norm_ring(usg_ctx *c) {
@@ -1715,12 +1771,20 @@ static void launch_norm_ring(usg_ctx *c) {
   if (c->ncase > 1) hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c->stream, c->d_ctr, 1);
 }
 
+// The norm rides on the sweep launches for a single case whose sweeps use
+// kBlock-thread blocks and have at least two forward launches.
+static bool fuse_norm(const usg_ctx *c) {
+  const int nw = (int)c->hm.fwd_poff.size() / (kXcd + 1);
+  return c->ncase == 1 && c->blk[USG_K_SWEEP] == kBlock && nw >= 3;
+}
+
 // One LU-SGS iteration (LUSGSIntegrator::UpdateSerial): residual, norm into
 // the ring, sweeps (the backward sweep writes the update), buffer swap.
 static void one_iteration(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   launch_residual_events(c, ea, eb);
-  launch_norm_ring(c);
-  launch_sweeps_any(c);
+  const bool fuse = fuse_norm(c);
+  if (!fuse) launch_norm_ring(c);
+  launch_sweeps_any(c, fuse);
   swap_state(c);
 }
 
