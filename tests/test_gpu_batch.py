@@ -3,7 +3,10 @@
 A batch is a block-diagonal union of independent cases, so every case must
 reproduce a separate single-case context bit for bit: state, per-case
 residual history and forces.  Single-case contexts are themselves pinned to
-the CPU oracle by test_gpu_parity.py.
+the CPU oracle by test_gpu_parity.py.  The residual history also pins the
+two launch forms of the norm: a single case runs it as blocks appended to its
+sweep launches, a batch as separate kernels (the same code on the same
+cells).
 """
 import numpy as np
 import pytest
