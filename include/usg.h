@@ -169,6 +169,19 @@ int usg_get_geometry(usg_ctx *ctx, double *vol, double *cc, double *fn, double *
 int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *fn, double *fa,
                    double *fc, double *wdist, int *nfwd, int *nbwd);
 
+/* Host-only colour-blocked renumbering, so that the LU-SGS sweeps (which
+ * follow the cell numbering, LUSGS_Integrator.cpp:492-561) run one wave per
+ * colour: 3 or 4 waves where a file numbering gives hundreds of levels.
+ * Cells are coloured (three colours, a fourth for a few cells) and sorted by
+ * (colour, spatial part, Hilbert key); faces get lC < rC (node order flipped
+ * with them) and are sorted by (part of lC, lC, rC).  Outputs, caller
+ * allocated: perm[nc] (new cell -> old cell), the renumbered faces
+ * fnode[2*nf], flc[nf], frc[nf] (boundary tags kept), and the number of
+ * colours.  Node coordinates are unchanged.  The same permutation as
+ * tmp_amd/mesh.py prepare().  No counterpart in the reference (which sweeps
+ * its file numbering); a renumbered mesh is an ordinary mesh for it. */
+int usg_mesh_renumber(const usg_mesh_desc *mesh, int *perm, int *fnode, int *flc, int *frc, int *ncolours);
+
 /* LU-SGS iterations run as replays of a captured two-iteration hipGraph
  * (default on); off = eager launches.  Results are identical either way. */
 int usg_use_graphs(usg_ctx *ctx, int on);

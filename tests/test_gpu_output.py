@@ -99,3 +99,25 @@ def test_driver_writes_reference_outputs(tmp_path):
     a = np.array([[float(t) for t in l.split("\t")] for l in v1])
     b = np.array([[float(t) for t in l.split("\t")] for l in v2])
     assert np.allclose(a, b, rtol=1e-5, atol=1e-9)
+
+
+def test_driver_renumbered_run_matches_oracle():
+    """usg_run --renumber: UnstructReader::renumber (usg_mesh_renumber) then
+    LU-SGS; the oracle on mesh.prepare's numbering of the same file (the
+    same permutation) gives the same residual and the same state of the
+    file's cell 0."""
+    exe = os.path.join(os.path.dirname(usg.LIB_PATH), "usg_run")
+    out = subprocess.run([exe, "--renumber", os.path.join(GOLDEN, "part2.grd"), "-2", "-2147483647", "Euler2D",
+                          "LUSGS", "0.5", "1e6", "2.0", "10", "1e-30"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    line = [l for l in out.stdout.splitlines() if l.startswith("iterations")][0]
+    res = float(line.split()[3])
+    u0 = [float(x.strip("(),")) for x in line.split("=")[1].split()]
+    m, perm, ncol = usg.mesh_renumber(part2())
+    assert ncol <= 4
+    orc = O.OracleSolver(O.OracleMesh(m), O.EULER, 0.5, 2.0, 1e6)
+    for _ in range(10):
+        orc.lusgs_step()
+    assert abs(res - orc.res_l2()) <= 1e-12 * orc.res_l2()
+    c0 = int(np.nonzero(perm == 0)[0][0])
+    assert np.allclose(u0, orc.U[c0], rtol=1e-12, atol=0)

@@ -12,12 +12,12 @@ import pytest
 
 from tmp_amd import mesh, usg
 
-from .common import part2
+from .common import one_quad, part2, two_cells
 
 MESHES = [("airfoil", lambda: mesh.airfoil_hybrid(ni=64, dy1=1e-4, rfar=20.0)),
           ("bump", lambda: mesh.channel_bump(seed=1001)),
           ("rect-tri", lambda: mesh.rect(12, 10, lx=2.0, ly=1.5, jitter=0.15, seed=3, triangles=True)),
-          ("part2", part2)]
+          ("part2", part2), ("two-cells", two_cells), ("one-quad", one_quad)]
 
 
 def _colours(m):
@@ -71,3 +71,30 @@ def test_plain_greedy_would_need_five_colours():
         used = {col[o] for o in nbl[v] if col[o] >= 0}
         col[v] = min(k for k in range(6) if k not in used)
     assert max(col) >= 3
+
+
+@pytest.mark.parametrize("name,mk", MESHES, ids=[x[0] for x in MESHES])
+def test_library_renumbering_equals_prepare(name, mk):
+    """usg_mesh_renumber (C++) and mesh.prepare (Python) give the same
+    numbering: same faces in the same order, same colour count; perm maps
+    new cells to old ones (the cell areas follow it)."""
+    m0 = mk()
+    mp, ncp = mesh.prepare(m0)
+    mc, perm, ncc = usg.mesh_renumber(m0)
+    assert ncc == ncp
+    assert np.array_equal(mc.fnode, mp.fnode)
+    assert np.array_equal(mc.flc, mp.flc) and np.array_equal(mc.frc, mp.frc)
+    assert np.array_equal(np.sort(perm), np.arange(m0.nc))
+    v0 = usg.mesh_geometry(m0)["vol"]
+    v1 = usg.mesh_geometry(mc)["vol"]
+    assert np.allclose(v1, v0[perm], rtol=1e-12, atol=0)
+
+
+def test_library_renumbering_rejects_bad_input():
+    z = np.zeros(0, np.int32)
+    with pytest.raises(usg.UsgError, match="empty mesh"):
+        usg.mesh_renumber(mesh.Mesh(np.zeros((0, 2)), np.zeros((0, 2), np.int32), z, z, 0, "empty"))
+    m = part2()
+    bad = mesh.Mesh(m.xy, m.fnode, m.flc, np.where(m.frc >= 0, m.nc + 5, m.frc).astype(np.int32), m.nc, "bad")
+    with pytest.raises(usg.UsgError, match="out of range"):
+        usg.mesh_renumber(bad)

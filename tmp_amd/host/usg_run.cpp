@@ -5,11 +5,15 @@
 // Tecplot cell data and Cp files into the working directory every
 // outputEvery steps (the reference: 5000).
 //
-//   usg_run <mesh.grd> <wallTag> <farTag> <Euler2D|ViscousFlow2D|SST2D>
+//   usg_run [--renumber] <mesh.grd> <wallTag> <farTag> <Euler2D|ViscousFlow2D|SST2D>
 //           <LUSGS|RungeKutta5> <Ma> <Re> <AoA> <maxStep> [minRes] [outputEvery]
+//
+// --renumber applies the colour-blocked renumbering (usg_mesh_renumber)
+// first: the sweeps then run a few waves instead of the file numbering's
+// levels.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
-#include <algorithm>
 #include <cstdlib>
 #include <string>
 #include <vector>
@@ -80,10 +84,17 @@ int main(int argc, char **argv) {
       return 1;
     }
   }
+  // --renumber: colour-blocked renumbering of the mesh before the run; the
+  // printed state is then that of the file's cell 0
+  const bool renumber = argc > 1 && std::string(argv[1]) == "--renumber";
+  if (renumber) {
+    argv++;
+    argc--;
+  }
   if (argc < 10) {
     std::fprintf(stderr,
-                 "usage: %s mesh.grd wallTag farTag Euler2D|ViscousFlow2D|SST2D LUSGS|RungeKutta5 Ma Re AoA "
-                 "maxStep [minRes] [outputEvery]\n",
+                 "usage: %s [--renumber] mesh.grd wallTag farTag Euler2D|ViscousFlow2D|SST2D LUSGS|RungeKutta5 Ma "
+                 "Re AoA maxStep [minRes] [outputEvery]\n",
                  argv[0]);
     return 2;
   }
@@ -94,6 +105,11 @@ int main(int argc, char **argv) {
     rd.setOverSign(std::atoi(argv[3]));
     rd.addFile(argv[1]);
     rd.readAll();
+    int c0 = 0;
+    if (renumber) {
+      const std::vector<int> perm = rd.renumber(0);
+      c0 = (int)(std::find(perm.begin(), perm.end(), 0) - perm.begin());
+    }
     double aoa = std::atof(argv[8]);
     double minres = argc > 10 ? std::atof(argv[10]) : 1e-8;
     usg::WrapSolver w;
@@ -106,9 +122,10 @@ int main(int argc, char **argv) {
     double cl, cd;
     w.flow()->preprocessAdvanceSerial(0);
     w.flow()->forces(0, cl, cd);
-    double **U = w.flow()->getU();
+    double **U = w.flow()->getU();  // U[mesh][4 * cell + var]
     std::printf("iterations %d  residual %.15e  CL %.15e  CD %.15e  U[0] = (%.17e %.17e %.17e %.17e)\n", n,
-                w.advance()->lastResidual(), cl, cd, U[0][0], U[0][1], U[0][2], U[0][3]);
+                w.advance()->lastResidual(), cl, cd, U[0][4 * c0], U[0][4 * c0 + 1], U[0][4 * c0 + 2],
+                U[0][4 * c0 + 3]);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "error: %s\n", e.what());
     return 1;

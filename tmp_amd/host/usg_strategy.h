@@ -56,6 +56,22 @@ class UnstructReader {
     meshes_.clear();
     for (const auto &f : files_) meshes_.push_back(read(f));
   }
+  // Colour-blocked renumbering of mesh i (usg_mesh_renumber), so that the
+  // LU-SGS sweeps run one wave per colour instead of one per level of the
+  // file numbering; returns perm[new cell] = file cell.  No counterpart in
+  // the reference, which sweeps its file numbering.
+  std::vector<int> renumber(int i) {
+    MeshData &m = meshes_.at(i);
+    const usg_mesh_desc d = m.desc();
+    std::vector<int> perm(m.nc), fn(2 * (size_t)m.nf), l(m.nf), r(m.nf);
+    int ncolours = 0;
+    if (usg_mesh_renumber(&d, perm.data(), fn.data(), l.data(), r.data(), &ncolours) != USG_OK)
+      throw std::runtime_error(usg_last_error());
+    m.fnode.swap(fn);
+    m.flc.swap(l);
+    m.frc.swap(r);
+    return perm;
+  }
   // CobaltReader::make_movement (reader/CobaltReader.h:87-93): translate mesh i
   void make_movement(int i, double dx, double dy, double /*dz*/) {
     MeshData &m = meshes_.at(i);

@@ -51,7 +51,7 @@ class _Params(C.Structure):
 
 EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "usg_init_freestream",
            "usg_set_prim", "usg_residual", "usg_lusgs_step", "usg_rk5_step", "usg_get_field", "usg_res_l2",
-           "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_timing_enable",
+           "usg_forces", "usg_sweep_waves", "usg_get_geometry", "usg_mesh_build", "usg_mesh_renumber", "usg_timing_enable",
            "usg_kernel_time", "usg_stencil_apply", "usg_copy_kernel", "usg_use_graphs", "usg_last_error",
            "usg_batch_create", "usg_case_count", "usg_set_block", "usg_get_block", "usg_autotune",
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
@@ -142,6 +142,20 @@ def mesh_geometry(m: _mesh.Mesh) -> dict:
                                               ("vol", "cc", "fn", "fa", "fc", "wdist")), C.byref(a), C.byref(b)))
     out["nfwd"], out["nbwd"] = a.value, b.value
     return out
+
+
+def mesh_renumber(m: _mesh.Mesh) -> tuple[_mesh.Mesh, np.ndarray, int]:
+    """Colour-blocked renumbering by the library (usg_mesh_renumber):
+    (renumbered mesh, perm[nc] new -> old cell, number of colours)."""
+    d, keep = _desc(m)
+    perm = np.zeros(m.nc, np.int32)
+    fnode = np.zeros((m.nf, 2), np.int32)
+    flc = np.zeros(m.nf, np.int32)
+    frc = np.zeros(m.nf, np.int32)
+    ncol = C.c_int()
+    _check(lib().usg_mesh_renumber(C.byref(d), perm.ctypes.data_as(_ip), fnode.ctypes.data_as(_ip),
+                                   flc.ctypes.data_as(_ip), frc.ctypes.data_as(_ip), C.byref(ncol)))
+    return _mesh.Mesh(m.xy.copy(), fnode, flc, frc, m.nc, m.name), perm, ncol.value
 
 
 def cell_nodes(m: _mesh.Mesh) -> tuple[np.ndarray, np.ndarray]:
