@@ -50,7 +50,9 @@ class _Solver(C.Structure):
                                       "mut_face", "tFc", "tFv", "tR", "tD", "tW", "tdW1", "tdW"]]
                 + [(n, C.c_double) for n in ["fs_mut", "fsnd_mut", "fsnd_mu"]]
                 + [("fs_turb", C.c_double * 2), ("fsnd_turb", C.c_double * 2)]
-                + [("hllc_pstar_violations", C.c_long), ("Fv_face", _dp)])
+                + [("hllc_pstar_violations", C.c_long), ("Fv_face", _dp), ("Fv_all", _dp)]
+                + [("dt_phys", C.c_double), ("dt_order", C.c_int), ("dt_steps", C.c_int),
+                   ("dts", C.c_double * 3), ("Wt1", _dp), ("Wt2", _dp), ("blank", _ip)])
 
 
 def build(force: bool = False) -> str:
@@ -85,6 +87,11 @@ def lib():
         L.orc_dw_rho_sq.restype = C.c_double
         L.orc_dw_rho_sq.argtypes = [C.POINTER(_Solver)]
         L.orc_forces.argtypes = [C.POINTER(_Solver), _dp, _dp]
+        L.orc_sst_eddy_viscosity.argtypes = [C.POINTER(_Solver)]
+        L.orc_dual_time.argtypes = [C.POINTER(_Solver), C.c_double, C.c_int]
+        L.orc_dual_time_advance.argtypes = [C.POINTER(_Solver)]
+        L.orc_set_blank.argtypes = [C.POINTER(_Solver), _ip]
+        L.orc_set_cells.argtypes = [C.POINTER(_Solver), C.c_int, _ip, _dp]
         L.orc_hllc_face.argtypes = [_dp, _dp, _dp, C.c_double, _dp, _ip]
         L.orc_lusgs_sweeps.argtypes = [C.POINTER(_Solver)]
         L.orc_cell_volume.restype = C.c_double
@@ -180,6 +187,31 @@ class OracleSolver:
 
     def res_l2(self) -> float:
         return lib().orc_density_residual_l2(self.ptr)
+
+    def dual_time(self, dt_phys: float, order: int = 2):
+        """Physical time step dt_phys by BDF1 on the first step, then BDF``order``."""
+        lib().orc_dual_time(self.ptr, float(dt_phys), int(order))
+
+    def dual_time_advance(self):
+        lib().orc_dual_time_advance(self.ptr)
+
+    def set_blank(self, blank):
+        """Overset cell blanking (1 field, 0 hole, -1 receptor; None clears)."""
+        if blank is None:
+            lib().orc_set_blank(self.ptr, None)
+            return
+        b = np.ascontiguousarray(blank, np.int32)
+        lib().orc_set_blank(self.ptr, b.ctypes.data_as(_ip))
+
+    def set_cells(self, idx, U):
+        """Primitive states U[n, 4] of cells idx (W from U), e.g. overset receptors."""
+        idx = np.ascontiguousarray(idx, np.int32)
+        U = np.ascontiguousarray(U, np.float64).reshape(idx.size, 4)
+        lib().orc_set_cells(self.ptr, idx.size, idx.ctypes.data_as(_ip), U.ctypes.data_as(_dp))
+
+    def eddy_viscosity(self):
+        """mu_t of the current state (tW = rho tU, the last residual's F2)."""
+        lib().orc_sst_eddy_viscosity(self.ptr)
 
     def forces(self):
         cl, cd = C.c_double(), C.c_double()

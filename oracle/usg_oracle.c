@@ -321,6 +321,8 @@ orc_solver *orc_solver_create(orc_mesh *m, int flow, double ma, double aoa_deg, 
   s->tFc = A(nf * 2); s->tFv = A(nf * 2); s->tR = A(nc * 2);
   s->tD = A(nc * 2); s->tW = A(nc * 2); s->tdW1 = A(nc * 2); s->tdW = A(nc * 2);
   s->Fv_face = A(nf * 2);
+  s->Fv_all = A(nf * NE);
+  s->Wt1 = A(nc * NE); s->Wt2 = A(nc * NE);
 #undef A
   /* LUSGS_Integrator.h:18 ; RungeKuttaStrategy.h:122-131 */
   s->cfl_lusgs = 400;
@@ -339,8 +341,9 @@ void orc_solver_destroy(orc_solver *s) {
                     s->Umax, s->Umin, s->phi, s->UL, s->UR, s->mu, s->mu_edge, s->D,
                     s->W, s->dW1, s->dW, s->W0, s->Wnow, s->tU, s->tUedge, s->tgrad,
                     s->F1, s->F2, s->mut, s->mut_face, s->tFc, s->tFv, s->tR, s->tD,
-                    s->tW, s->tdW1, s->tdW, s->Fv_face};
+                    s->tW, s->tdW1, s->tdW, s->Fv_face, s->Fv_all, s->Wt1, s->Wt2};
   for (size_t i = 0; i < sizeof(arrs) / sizeof(arrs[0]); i++) free(arrs[i]);
+  free(s->blank);
   free(s);
 }
 
@@ -845,6 +848,7 @@ static void laminar_flux(orc_solver *s) {
     Fv[3] = (taoxy * n0 + taoyy * n1) * A;
     s->Fv_face[2 * k] = Fv[2];
     s->Fv_face[2 * k + 1] = Fv[3];
+    for (int j = 0; j < NE; j++) s->Fv_all[NE * k + j] = Fv[j];
     for (int j = 0; j < NE; j++) s->R[j + NE * lC] -= Fv[j];
     if (rC >= 0)
       for (int j = 0; j < NE; j++) s->R[j + NE * rC] += Fv[j];
@@ -1041,6 +1045,69 @@ static void sst_total_flux(orc_solver *s) {
   }
 }
 
+/* Dual time stepping (no counterpart in the reference, whose iterations are
+ * pseudo-time only): the physical-time derivative of W by backward
+ * differences, V (c0 W + c1 W^n + c2 W^(n-1)) with c = (1, -1, 0) / dt on
+ * the first physical step and (3/2, -2, 1/2) / dt after it (order 2), added
+ * to each cell's residual after its face fluxes. */
+static void dual_time_residual(orc_solver *s) {
+  const orc_mesh *m = s->m;
+  for (int k = 0; k < m->nc; k++) {
+    const double V = m->vol[k];
+    for (int j = 0; j < NE; j++) {
+      int i = j + NE * k;
+      s->R[i] += V * (s->dts[0] * s->W[i] + s->dts[1] * s->Wt1[i] + s->dts[2] * s->Wt2[i]);
+    }
+  }
+}
+
+void orc_dual_time(orc_solver *s, double dt_phys, int order) {
+  s->dt_phys = dt_phys;
+  s->dt_order = order;
+  s->dt_steps = 0;
+  s->dts[0] = s->dts[1] = s->dts[2] = 0.0;
+}
+
+/* A new physical step: the current state becomes W^n (and W^n becomes
+ * W^(n-1)). */
+void orc_dual_time_advance(orc_solver *s) {
+  const int n = s->m->nc * NE;
+  s->dt_steps++;
+  const int bdf2 = s->dt_order == 2 && s->dt_steps > 1;
+  if (bdf2) memcpy(s->Wt2, s->Wt1, sizeof(double) * n);
+  memcpy(s->Wt1, s->W, sizeof(double) * n);
+  const double dt = s->dt_phys;
+  if (bdf2) {
+    s->dts[0] = 1.5 / dt;
+    s->dts[1] = -2.0 / dt;
+    s->dts[2] = 0.5 / dt;
+  } else {
+    s->dts[0] = 1.0 / dt;
+    s->dts[1] = -1.0 / dt;
+    s->dts[2] = 0.0;
+  }
+}
+
+static int frozen(const orc_solver *s, int k) { return s->blank && s->blank[k] != 1; }
+
+void orc_set_blank(orc_solver *s, const int *blank) {
+  free(s->blank);
+  s->blank = NULL;
+  if (!blank) return;
+  s->blank = (int *)xcalloc(s->m->nc, sizeof(int));
+  memcpy(s->blank, blank, sizeof(int) * s->m->nc);
+}
+
+void orc_set_cells(orc_solver *s, int n, const int *idx, const double *U) {
+  for (int i = 0; i < n; i++) {
+    int k = idx[i];
+    for (int j = 0; j < NE; j++) s->U[j + NE * k] = U[j + NE * i];
+    double W[NE];
+    orc_prim_to_cons(&s->U[NE * k], W);
+    for (int j = 0; j < NE; j++) s->W[j + NE * k] = W[j];
+  }
+}
+
 /* preprocessAdvanceSerial: Euler2D.h:133-140, ViscousFlow2D.h:192-212; for SST
  * the turbulence residual is inserted where ViscousFlow2D::preprocessAdvance
  * (:163-190) calls SolveTurbulenceEquation. */
@@ -1061,6 +1128,7 @@ void orc_residual(orc_solver *s) {
   solve_spectral_radius(s);
   hllc_compute_flux(s);
   if (!is_inviscid(s)) laminar_flux(s);
+  if (s->dts[0] != 0.0) dual_time_residual(s);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1085,6 +1153,7 @@ static void solve_diag(orc_solver *s) {
   for (int k = 0; k < m->nc; k++) {
     double diag = m->vol[k] / s->dt[k] + 0.5 * OMEGAN * s->lamc[k];
     if (!is_inviscid(s)) diag += s->lamv[k];
+    if (s->dts[0] != 0.0) diag += s->dts[0] * m->vol[k];   /* dual time */
     for (int j = 0; j < NE; j++) s->D[j + NE * k] = diag;
   }
 }
@@ -1227,6 +1296,8 @@ static void forward_sweep(orc_solver *s) {
       for (int l = 0; l < NE; l++) dFi[l] += dF[l];
     }
     for (int j = 0; j < NE; j++) s->dW1[j + NE * k] = (-s->R[j + NE * k] - dFi[j]) / s->D[j + NE * k];
+    if (frozen(s, k))
+      for (int j = 0; j < NE; j++) s->dW1[j + NE * k] = 0.0;
   }
 }
 
@@ -1240,6 +1311,8 @@ static void backward_sweep(orc_solver *s) {
       for (int l = 0; l < NE; l++) dFi[l] += dF[l];
     }
     for (int j = 0; j < NE; j++) s->dW[j + NE * k] = s->dW1[j + NE * k] - dFi[j] / s->D[j + NE * k];
+    if (frozen(s, k))
+      for (int j = 0; j < NE; j++) s->dW[j + NE * k] = 0.0;
   }
 }
 
@@ -1271,6 +1344,16 @@ static void sst_backward_sweep(orc_solver *s) {
   }
 }
 
+/* mu_t = a1 rho k / max(a1 omega, F2 |curl u|) from the conservative
+ * (rho k, rho omega) and the last residual's velocity gradient and F2
+ * (SSTKWStrategy.h:194-223, ComputeTurbulenceViscousCoefficient :725-780) */
+static double eddy_viscosity(const orc_solver *s, int k) {
+  double rho = s->U[NE * k];
+  double curl = fabs(s->gradU[2 * (3 + NE * k)] - s->gradU[2 * (2 + NE * k) + 1]);
+  double h = smax(sst_a1 * s->tW[2 * k + 1] / rho, s->F2[k] * curl);
+  return sst_a1 * s->tW[2 * k] / h;
+}
+
 /* SSTkomegaModel::UpdatePrimitiveVariables (SSTKWStrategy.h:194-223) */
 static void sst_update(orc_solver *s) {
   for (int k = 0; k < s->m->nc; k++) {
@@ -1290,10 +1373,15 @@ static void sst_update(orc_solver *s) {
         s->tW[2 * k + q] = rho * fl;
       }
     }
-    double curl = fabs(s->gradU[2 * (3 + NE * k)] - s->gradU[2 * (2 + NE * k) + 1]);
-    double h = smax(sst_a1 * s->tW[2 * k + 1] / rho, s->F2[k] * curl);
-    s->mut[k] = sst_a1 * s->tW[2 * k] / h;
+    s->mut[k] = eddy_viscosity(s, k);
   }
+}
+
+/* SST eddy viscosity of the current state, as sst_update sets it: for
+ * tests, which set U, tU, tW = rho tU and run the residual first (for the
+ * gradients and F2). */
+void orc_sst_eddy_viscosity(orc_solver *s) {
+  for (int k = 0; k < s->m->nc; k++) s->mut[k] = eddy_viscosity(s, k);
 }
 
 /* LUSGSIntegrator::UpdateSerial (LUSGS_Integrator.cpp:228-255) */
@@ -1308,9 +1396,21 @@ void orc_lusgs_step(orc_solver *s) {
     sst_forward_sweep(s);
     sst_backward_sweep(s);
   }
-  int n = s->m->nc * NE;
-  for (int k = 0; k < n; k++) s->W[k] = s->W[k] + s->dW[k];
-  prim_from_cons(s, s->W, s->U);
+  if (!s->blank) {
+    int n = s->m->nc * NE;
+    for (int k = 0; k < n; k++) s->W[k] = s->W[k] + s->dW[k];
+    prim_from_cons(s, s->W, s->U);
+  } else {
+    /* overset: hole and receptor cells keep their state as it is */
+    for (int k = 0; k < s->m->nc; k++) {
+      if (frozen(s, k)) continue;
+      for (int j = 0; j < NE; j++) s->W[j + NE * k] = s->W[j + NE * k] + s->dW[j + NE * k];
+      double Wk[NE], Uk[NE];
+      for (int j = 0; j < NE; j++) Wk[j] = s->W[j + NE * k];
+      orc_cons_to_prim(Wk, Uk);
+      for (int j = 0; j < NE; j++) s->U[j + NE * k] = Uk[j];
+    }
+  }
   if (s->flow == ORC_SST) sst_update(s);
 }
 
@@ -1366,6 +1466,7 @@ void orc_rk5_step(orc_solver *s) {
 double orc_density_residual_l2(orc_solver *s) {
   double acc = 0;
   for (int k = 0; k < s->m->nc; k++) {
+    if (frozen(s, k)) continue;
     double r = s->R[NE * k] / s->m->vol[k];
     acc += r * r;
   }

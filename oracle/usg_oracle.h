@@ -71,6 +71,17 @@ typedef struct orc_solver {
   long hllc_pstar_violations;
   /* viscous momentum flux of each face (tau . n A), for the wall force */
   double *Fv_face;
+  /* the whole viscous face flux (0, (u.tau - q).n A, tau.n A) [nf*4], for tests */
+  double *Fv_all;
+  /* dual time stepping (no counterpart in the reference): physical step,
+   * order, steps taken, BDF coefficients / dt, earlier time levels [nc*4] */
+  double dt_phys;
+  int dt_order, dt_steps;
+  double dts[3];
+  double *Wt1, *Wt2;
+  /* overset cell blanking (NULL: every cell a field cell): 1 field, 0 hole,
+   * -1 receptor; only field cells are updated and enter the norm */
+  int *blank;
 } orc_solver;
 
 /* mesh / geometry (edge3d_int.h, cell3d_int.h, cellVolume.cpp, CellBrick.cpp) */
@@ -89,6 +100,13 @@ void orc_rk5_step(orc_solver *s);                /* RK5 singleStepSerial        
 double orc_density_residual_l2(orc_solver *s);   /* ||R_rho / V||_2             */
 double orc_dw_rho_sq(orc_solver *s);             /* sum dW_rho^2 (postprocessUpdate) */
 void orc_forces(orc_solver *s, double *cl, double *cd);
+void orc_sst_eddy_viscosity(orc_solver *s);      /* mu_t of the current state */
+/* dual time stepping: BDF1 on the first physical step, then BDF2 (order 2) */
+void orc_dual_time(orc_solver *s, double dt_phys, int order);
+void orc_dual_time_advance(orc_solver *s);
+/* overset: cell blanking (NULL clears) and receptor states U[n*4] of cells idx[n] */
+void orc_set_blank(orc_solver *s, const int *blank);
+void orc_set_cells(orc_solver *s, int n, const int *idx, const double *U);
 /* standalone pieces for invariant tests */
 void orc_lusgs_sweeps(orc_solver *s);
 double orc_cell_volume(const double *xy, int nv);

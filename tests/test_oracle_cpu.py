@@ -138,7 +138,7 @@ def test_sutherland_at_freestream():
 # residual invariants
 # --------------------------------------------------------------------------
 
-@pytest.mark.parametrize("flow", [O.EULER, O.LAMINAR])
+@pytest.mark.parametrize("flow", [O.EULER, O.LAMINAR, O.SST])
 def test_freestream_preservation(flow):
     """A uniform free stream is steady: on a far-field box every residual is
     roundoff, the gradient vanishes and the limiter stays 1 (catches a wrong

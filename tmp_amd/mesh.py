@@ -21,6 +21,7 @@ oracle and the GPU solver are always run on the same renumbered mesh.
 from __future__ import annotations
 
 import dataclasses
+import gzip
 import heapq
 import math
 import os
@@ -76,9 +77,11 @@ def read_grd(path: str, wall_tag: int, far_tag: int) -> Mesh:
 
     ``wall_tag`` and ``far_tag`` are the file's negative patch tags for wall
     and far-field faces; any other negative tag raises ``ValueError`` as the
-    reference throws "Not a valid boundary type".
+    reference throws "Not a valid boundary type".  A path ending in ``.gz``
+    is read through gzip.
     """
-    with open(path, "r") as fh:
+    opener = gzip.open if path.endswith(".gz") else open
+    with opener(path, "rt") as fh:
         tok = fh.read().split()
     ndim = int(tok[0])
     if ndim != 2:
@@ -104,7 +107,7 @@ def read_grd(path: str, wall_tag: int, far_tag: int) -> Mesh:
         raise ValueError("Not a valid boundary type")
     frc[wall] = WALL
     frc[far] = FARFIELD
-    return Mesh(xy, fnode, (lc - 1).astype(np.int32), frc, nc, os.path.basename(path))
+    return Mesh(xy, fnode, (lc - 1).astype(np.int32), frc, nc, os.path.basename(path).removesuffix(".gz"))
 
 
 def write_grd(path: str, m: Mesh, wall_tag: int = -1, far_tag: int = -2) -> None:
