@@ -231,6 +231,21 @@ int usg_stencil_apply(const void *dev_src, void *dev_dst, int nvar, int layout, 
  * solution arrays registered with registerSolution in place. */
 int usg_stencil_apply_host(const double *src, double *dst, int nvar, int layout, long src_pts, long dst_pts,
                            int nrecv, const int *recv, const int *nw, const int *donor, const double *w);
+/* usg_stencil_apply_host without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate GPU memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The device copies of the arrays are staged in a fixed 16 MiB
+ * array that the library holds on each GPU it runs on until the process
+ * ends; calls whose arrays need more than that (src and dst at
+ * src_pts*nvar and dst_pts*nvar doubles, 24 bytes of indices and 32 bytes of
+ * weights per receptor, each part rounded up to 256 bytes) return
+ * USG_ERR_ARG.  It calls no hipMalloc/hipFree.  Every call on a GPU shares
+ * that GPU's staging area, so calls that overlap on one GPU (from several
+ * host threads) race; serialise them.  Results equal usg_stencil_apply_host's
+ * bit for bit. */
+int usg_stencil_apply_host_noalloc(const double *src, double *dst, int nvar, int layout, long src_pts,
+                                   long dst_pts, int nrecv, const int *recv, const int *nw, const int *donor,
+                                   const double *w);
 /* Donor weights of n receptor points (host arrays): point r lies in a cell
  * of nvert[r] (3 or 4) vertices xv[r*8 .. r*8+2*nvert[r]) and gets
  * w[r*4 .. r*4+nvert[r]) -- computeNodalWeight2D (mathTools/math.cpp:229-312):

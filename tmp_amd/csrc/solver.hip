@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2233,6 +2234,77 @@ int usg_stencil_apply_host(const double *src, double *dst, int nvar, int layout,
     return USG_OK;
   }
   return fail(USG_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Staging area of usg_stencil_apply_host_noalloc: the device copies of its
+// host arrays live in this fixed array instead of hipMalloc'd buffers.  Each
+// GPU holds its own copy at its own address, found once per device with
+// hipGetSymbolAddress; host code uses only that address (the array's name is
+// a host-side stand-in).  Calls that overlap on one GPU share it and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(void **out) {
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(USG_ERR_ARG, "device index beyond the staging table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(USG_ERR_HIP, "hipGetSymbolAddress returned a null staging address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return USG_OK;
+}
+
+static size_t seg256(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+extern "C" {
+
+int usg_stencil_apply_host_noalloc(const double *src, double *dst, int nvar, int layout, long src_pts, long dst_pts,
+                                   int nrecv, const int *recv, const int *nw, const int *donor, const double *w) {
+  if (!src || !dst || nvar <= 0 || nrecv < 0 || src_pts < 0 || dst_pts < 0 ||
+      (layout != USG_ROW && layout != USG_COLUMN) || (nrecv && (!recv || !nw || !donor || !w)))
+    return fail(USG_ERR_ARG, "bad argument");
+  if (src_pts > 2147483647L || dst_pts > 2147483647L) return fail(USG_ERR_ARG, "point count exceeds int32");
+  if (nrecv == 0) return USG_OK;
+  const size_t ns = (size_t)src_pts * nvar, nd = (size_t)dst_pts * nvar, nr = (size_t)nrecv;
+  // layout of the staging area: src | dst | recv | nw | donor | w, each 256-byte aligned
+  const size_t o_dst = seg256(ns * sizeof(double));
+  const size_t o_recv = o_dst + seg256(nd * sizeof(double));
+  const size_t o_nw = o_recv + seg256(nr * sizeof(int));
+  const size_t o_donor = o_nw + seg256(nr * sizeof(int));
+  const size_t o_w = o_donor + seg256(nr * 4 * sizeof(int));
+  const size_t total = o_w + nr * 4 * sizeof(double);
+  if (total > kScratchBytes)
+    return fail(USG_ERR_ARG, "stencil arrays need " + std::to_string(total) +
+                                 " bytes, beyond the 16 MiB staging area of usg_stencil_apply_host_noalloc");
+  void *base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  unsigned char *b = (unsigned char *)base;
+  double *ds = (double *)b, *dd = (double *)(b + o_dst), *dw = (double *)(b + o_w);
+  int *dr = (int *)(b + o_recv), *dn = (int *)(b + o_nw), *ddn = (int *)(b + o_donor);
+  HIPCHK(hipMemcpy(ds, src, ns * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dd, dst, nd * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dr, recv, nr * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dn, nw, nr * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ddn, donor, nr * 4 * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dw, w, nr * 4 * sizeof(double), hipMemcpyHostToDevice));
+  int rc = usg_stencil_apply(ds, dd, nvar, layout, src_pts, dst_pts, nrecv, dr, dn, ddn, dw, nullptr);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(nullptr));
+  HIPCHK(hipMemcpy(dst, dd, nd * sizeof(double), hipMemcpyDeviceToHost));
+  return USG_OK;
 }
 
 int usg_copy_kernel(void *dst, const void *src, long bytes, void *st) {

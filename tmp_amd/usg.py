@@ -57,7 +57,7 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
            "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state", "usg_overset_create",
            "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution",
-           "usg_overset_median", "usg_overset_obb", "usg_stencil_apply_host"]
+           "usg_overset_median", "usg_overset_obb", "usg_stencil_apply_host", "usg_stencil_apply_host_noalloc"]
 
 _lib = None
 
@@ -113,6 +113,7 @@ def lib():
         L.usg_overset_median.argtypes = [_ip, _dp, C.c_int]
         L.usg_stencil_apply_host.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_long, C.c_long, C.c_int, _ip, _ip, _ip,
                                              _dp]
+        L.usg_stencil_apply_host_noalloc.argtypes = L.usg_stencil_apply_host.argtypes
         L.usg_overset_obb.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
