@@ -25,6 +25,9 @@ WORKLOADS = {
     "bump20k": ("bump", 0, 0.5, 0.0, 1e6, "Euler FP64, channel bump ~20k cells"),
     # 64 NACA 4-digit cases (seed 4004), per-case Mach and alpha, one batched context
     "batch64": ("batch", 2, None, None, 6e6, "SST k-omega RANS FP64, 64-case NACA 4-digit batch (seed 4004)"),
+    # cylinder O-grid over a Cartesian background: overset coupling, dual time (BDF2, dt 0.5)
+    "c2": ("overset", 1, 0.2, 0.0, 100.0, "laminar NS FP64, cylinder O-grid 256x96 over background 400x200, "
+                                          "overset coupling, dual time"),
 }
 
 
@@ -54,6 +57,65 @@ def build_batch(workers=8):
     return cases, meshes
 
 
+def host_cpu():
+    """nproc and the CPU model name of this host (from /proc/cpuinfo)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def time_to_tolerance(n_train, n_test, epochs):
+    """The metric's second half: wall time to ||R_rho/V|| = 1e-8 of the
+    first residual on held-out laminar NACA 4-digit cases, for the plain
+    cold start (each case alone, from the free stream), the batched cold
+    start, and the batched NN warm start (inference and the cold-start
+    judge included), with the network trained in this run on fields this
+    solver converged (tmp_amd/warmstart.py)."""
+    from tmp_amd import warmstart
+    r = warmstart.experiment(n_train=n_train, n_test=n_test, epochs=epochs)
+    t = r["measured_seconds_to_1e-8"]
+    lv = r["levels"]["1e-08"]
+    return {
+        "family": r["family"], "train_cases": r["train_cases"], "test_cases": r["test_cases"],
+        "epochs": r["epochs"], "train_seconds": r["train_seconds"], "tolerance": "1e-8 x cold first residual",
+        "seconds_cold_plain": t["cold_plain_single_case_solves"], "seconds_cold_batched": t["cold_batched"],
+        "seconds_warm_batched": t["warm_batched_incl_inference_and_judge"],
+        "speedup_warm_batched_vs_cold_plain": r["measured_speedup_warm_batched_vs_cold_plain"],
+        "speedup_warm_batched_vs_cold_batched": r["measured_speedup_warm_batched_vs_cold_batched"],
+        "median_iteration_speedup": lv["median_iteration_speedup"],
+        "warm_kept": r["warm_kept"], "fell_back_to_cold": r["fell_back_to_cold"],
+        "max_abs_dCL": r["max_abs_dCL_converged"], "max_abs_dCD": r["max_abs_dCD_converged"],
+    }
+
+
+def mixed_error(solver, m, flow, mach, aoa, re):
+    """One residual of the mixed-precision solver's final state in both
+    precisions: max |R_mixed - R_fp64| / max |R_fp64| per equation (rho, rhoE,
+    rhou, rhov, and rho k, rho omega for SST).  Mixed precision moves the
+    near-cancelling k residual of SST states far more than the flow
+    equations, so this is reported beside the mixed throughput."""
+    import numpy as np
+    from tmp_amd import usg
+    ref = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re)
+    sst = flow == usg.SST
+    ref.set_state(solver.U, solver.field(usg.F_CONS), solver.field(usg.F_TURB) if sst else None,
+                  solver.field(usg.F_TCONS) if sst else None, solver.field(usg.F_MUT) if sst else None)
+    solver.residual()
+    ref.residual()
+    a, b = solver.R, ref.R
+    if sst:
+        a = np.concatenate([a, solver.field(usg.F_TRES)], axis=1)
+        b = np.concatenate([b, ref.field(usg.F_TRES)], axis=1)
+    ref.close()
+    return [float(x) for x in np.max(np.abs(a - b), axis=0) / np.maximum(np.max(np.abs(b), axis=0), 1e-300)]
+
+
 def cpu_baseline(m, flow, mach, aoa, re, seconds):
     """The oracle (C restatement of the reference's serial path, 1 core) on
     the same mesh and inputs, iterated until ``seconds`` of CPU work."""
@@ -71,6 +133,33 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
             break
     dt = time.perf_counter() - t0
     return m.nc * n / dt, n, dt
+
+
+def cpu_baseline_overset(seconds):
+    """The oracle on the overset pair: its two serial solvers coupled by
+    its stage-by-stage interpolation, dual time (BDF2, dt 0.5), sub-iterated
+    until ``seconds`` of CPU work (1 core)."""
+    from oracle import oracle as O
+    from oracle.overset_oracle import OversetCoupling
+    from tmp_amd import overset_c2
+    og, bg, conn, _ = overset_c2.build()
+    oms = [O.OracleMesh(og), O.OracleMesh(bg)]
+    cp = OversetCoupling(oms, conn)
+    ss = [O.OracleSolver(om, 1, overset_c2.MACH, 0.0, overset_c2.RE) for om in oms]
+    for s, b in zip(ss, cp.blank):
+        s.set_blank(b)
+        s.dual_time(0.5, 2)
+        s.dual_time_advance()
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        cp.apply(ss)
+        for s in ss:
+            s.lusgs_step()
+        n += 1
+        print(f"# cpu baseline sub-iteration {n}: {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    dt = time.perf_counter() - t0
+    return (og.nc + bg.nc) * n / dt, n, dt
 
 
 def _cpu_case(args):
@@ -92,11 +181,12 @@ def cpu_baseline_batch(cases, flow, re, seconds, workers):
     return sum(r[0] for r in res), sum(r[1] for r in res), len(sample)
 
 
-def measured_traffic(workload, nc):
-    """HBM bytes per k_face launch from the committed rocprofv3 --pmc summary
-    of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what it reads
-    for the plain copy kernel), or None when there is none for this mesh."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"k_face_{workload}_traffic.md")
+def measured_traffic(kernel, workload, nc):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what
+    it reads for the plain copy kernel), or None when there is none for this
+    mesh."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{kernel}_{workload}_traffic.md")
     if not os.path.exists(path):
         return None
     vals = {}
@@ -138,6 +228,8 @@ def main():
     ap.add_argument("--workload", default="sst1m", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-time-to-tol", action="store_true",
+                    help="skip the time-to-1e-8 comparison (NN warm vs plain cold) of the default workload")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed: FP32 gradients/limiter/fluxes, FP64 state, accumulation, diagonal and norms")
     ap.add_argument("--autotune", action="store_true",
@@ -158,7 +250,20 @@ def main():
     mixed = a.precision == "mixed"
     prec = usg.MIXED if mixed else usg.FP64
     t = time.perf_counter()
-    if kind == "batch":
+    if kind == "overset":
+        from tmp_amd import overset_c2
+        og, bg, _, st = overset_c2.build()
+        nc, nf = og.nc + bg.nc, og.nf + bg.nf
+        print(f"# overset pair: {og.nc} + {bg.nc} cells, {len(st['recv'])} receptor and "
+              f"{int((st['blank'] == 0).sum())} hole cells, built in {time.perf_counter() - t:.1f}s",
+              file=sys.stderr, flush=True)
+        solver = overset_c2.solver(og, bg, st, dt=0.5)
+        solver.dual_time_advance()
+        config = {"workload": label, "cells": nc, "faces": nf, "receptor_cells": int(len(st["recv"])),
+                  "hole_cells": int((st["blank"] == 0).sum()), "mach": mach, "reynolds": re,
+                  "dual_time": "BDF2, dt 0.5; steps are sub-iterations of one physical step"}
+        metric = "cell-iters/s, overset cylinder (two grids), dual-time LU-SGS"
+    elif kind == "batch":
         cases, meshes = build_batch()
         nc, nf = sum(m.nc for m in meshes), sum(m.nf for m in meshes)
         print(f"# batch of {len(meshes)} cases: {nc} cells, {nf} faces, built in {time.perf_counter() - t:.1f}s",
@@ -198,7 +303,7 @@ def main():
     t1 = time.perf_counter()
     elapsed = t1 - t0
     # timed region 2: the same iterations launched eagerly with HIP events
-    # around every launch of the dominant kernel (k_face) on its stream
+    # around every launch of the dominant kernel on its stream
     solver.timing(True)
     t2 = time.perf_counter()
     solver.lusgs(a.steps)
@@ -208,6 +313,10 @@ def main():
     config["monitors"] = {"res_l2_first": float(hist.reshape(a.steps, -1)[0, 0]),
                           "res_l2_last": float(hist.reshape(a.steps, -1)[-1, 0]),
                           "sum_dW_rho_sq": float(solver.dw_rho_sq()[0])}
+    if mixed and kind not in ("batch", "overset"):
+        config["mixed_vs_fp64_residual"] = mixed_error(solver, m, flow, mach, aoa, re)
+        print(f"# mixed vs FP64 residual of the final state: {config['mixed_vs_fp64_residual']}", file=sys.stderr,
+              flush=True)
     if kind == "batch":
         st = solver.status()
         config["failed_cases"] = int((st != 0).sum())
@@ -215,7 +324,10 @@ def main():
               flush=True)
     value = nc * a.steps / elapsed
     r = nf / nc
-    kbytes = roofline.face_kernel_bytes_per_cell(flow, r, mixed) * nc
+    # the dominant kernel: the fused face-cell launch in FP64, the face launch otherwise
+    kname = "k_facecell" if not mixed else "k_face"
+    kbytes = (roofline.fused_kernel_bytes_per_cell(flow, r) if not mixed
+              else roofline.face_kernel_bytes_per_cell(flow, r, mixed)) * nc
     kavg_s = ktime_ms / 1e3 / max(klaunch, 1)
     achieved = kbytes / kavg_s / 1e9
     it_frac = roofline.iteration_bytes_per_cell(flow, r) * value / 1e9 / roofline.HBM_PEAK_GBS
@@ -237,24 +349,38 @@ def main():
         "dtype": "f32+f64" if mixed else "f64",
         "data": "synthetic: generated mesh, free-stream cold start",
         "config": config,
-        "roofline": {"bound": "hbm", "kernel": "k_face", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": roofline.HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / roofline.HBM_PEAK_GBS,
-                     "traffic": None if mixed else measured_traffic(a.workload, nc),
+                     "traffic": None if mixed else measured_traffic(kname, a.workload, nc),
                      "iteration_frac": it_frac},
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        if kind == "batch":
-            workers = min(16, os.cpu_count() or 1)
+        nproc, model = host_cpu()
+        if kind == "overset":
+            v, n, dt = cpu_baseline_overset(a.cpu_seconds)
+            out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": 1, "kind": "port",
+                                   "nproc": nproc, "cpu_model": model,
+                                   "sample": f"{n} dual-time LU-SGS sub-iterations of the same two grids with the "
+                                             f"oracle's coupling ({dt:.1f}s, oracle C restatement, -O2, 1 thread)"}
+        elif kind == "batch":
+            # one case per core, at most 16: the CPU share of a one-GPU host
+            workers = min(16, nproc)
             v, n, k = cpu_baseline_batch(cases, flow, re, a.cpu_seconds, workers)
             out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": workers, "kind": "port",
+                                   "nproc": nproc, "cpu_model": model,
                                    "sample": f"first {k} batch cases, one per core, {n} LU-SGS iterations in "
                                              f"total over {a.cpu_seconds:.0f}s each (oracle C restatement, -O2)"}
         else:
             v, n, dt = cpu_baseline(m, flow, mach, aoa, re, a.cpu_seconds)
             out["cpu_baseline"] = {"value": v, "unit": "cell-iterations/s", "cores": 1, "kind": "port",
+                                   "nproc": nproc, "cpu_model": model,
                                    "sample": f"{n} LU-SGS iterations of the same {nc}-cell workload from the "
                                              f"free stream ({dt:.1f}s, oracle C restatement, -O2, 1 thread)"}
+    if a.workload == "sst1m" and not mixed and not a.no_time_to_tol:
+        solver.close()
+        print("# time to 1e-8: NN warm start vs plain cold start", file=sys.stderr, flush=True)
+        out["time_to_tolerance"] = time_to_tolerance(256, 8, 150)
     print(json.dumps(out), flush=True)
 
 if __name__ == "__main__":

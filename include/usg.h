@@ -36,7 +36,12 @@ extern "C" {
 /* precision of the residual kernels */
 #define USG_FP64 0  /* every kernel in FP64 (the drop-in)                      */
 #define USG_MIXED 1 /* gradients, limiter and fluxes in FP32; residual sums,
-                       LU-SGS diagonal inverse and norms in FP64               */
+                       LU-SGS diagonal inverse and norms in FP64.  Limit: the
+                       SST k residual is a near-cancellation of production and
+                       destruction; one FP32 evaluation moves it by up to 0.5
+                       of its largest value on the 1M-cell airfoil (the flow
+                       equations by < 1e-3), so use SST with USG_MIXED only
+                       where that error is acceptable (bench.py reports it). */
 
 /* fields readable with usg_get_field (host layout AoS, see each comment) */
 #define USG_F_PRIM 0     /* U   [nc*4]  (rho, p, u, v)     Euler2D::getU          */
@@ -57,6 +62,9 @@ extern "C" {
 #define USG_F_TDW 15     /* turbulence backward increment [nc*2]                   */
 #define USG_F_TCONS 16   /* (rho k, rho omega) [nc*2]       SSTkomegaModel W      */
 #define USG_F_W0 17      /* W at the start of the last RK5 step [nc*4]  W_scratch */
+#define USG_F_UEDGE 18   /* face states of the current U [nf*4]  getUEdge (computed on request) */
+#define USG_F_WDIST 19   /* SST: distance to the nearest wall face segment [nc] (the
+                            reference's projected distance is usg_get_geometry's wdist) */
 
 typedef struct usg_mesh_desc {
   int nn;              /* nodes                                              */
@@ -152,9 +160,11 @@ int usg_res_l2(usg_ctx *ctx, double *out);
 /* Lift and drag coefficients of the current state, cl[ncases], cd[ncases]:
  * the momentum flux out of the fluid through the wall faces -- p n A with
  * the wall cell's pressure, minus the viscous flux tau . n A for viscous
- * flow (which evaluates the residual of the current state first) --
- * rotated by the angle of attack and scaled by 1/2 gamma M^2.  The
- * reference integrates no forces; this is the definition used here. */
+ * flow, with the wall cell's velocity gradient and the face stress computed
+ * from the current state exactly as a residual evaluation would (no other
+ * field of the context changes) -- rotated by the angle of attack and
+ * scaled by 1/2 gamma M^2.  The reference integrates no forces; this is the
+ * definition used here. */
 int usg_forces(usg_ctx *ctx, double *cl, double *cd);
 
 /* Mesh facts: number of LU-SGS waves (forward, backward) of the numbering. */
@@ -185,6 +195,13 @@ int usg_mesh_renumber(const usg_mesh_desc *mesh, int *perm, int *fnode, int *flc
 /* LU-SGS iterations run as replays of a captured two-iteration hipGraph
  * (default on); off = eager launches.  Results are identical either way. */
 int usg_use_graphs(usg_ctx *ctx, int on);
+/* FP64 residuals compute the face fluxes and gather them into the cells in
+ * one launch, the face record held in LDS per tile of cells (default on);
+ * off = a face launch that writes the face record to HBM and a cell launch
+ * that reads it back.  Results are identical either way.  Fails with
+ * USG_ERR_ARG (and stays off) when a tile's face record would not fit the
+ * LDS of a block.  No counterpart in the reference (a launch choice). */
+int usg_use_fused(usg_ctx *ctx, int on);
 
 /* Threads per block of each kernel group (64, 128 or 256; default 256):
  * the gradient/limiter kernel, the face flux kernel, the cell gather kernel
@@ -326,6 +343,67 @@ int usg_overset_resolution(usg_overset *ov, int block, double *node_res, double 
  * axes vec[2*2] (row i = axis i). */
 int usg_overset_median(int *ix, double *x, int n);
 int usg_overset_obb(const double *x, int n, double *xc, double *dxc, double *vec);
+
+/* Cell-centred coupling of an overset assembly whose blocks are the cases
+ * of a batch context (usg_batch_create, blocks in order): the donor stencil
+ * of every receptor cell as one linear map of donor field cells, composed
+ * of the stages the reference strings together around tioga::dataUpdate --
+ * field-cell values averaged to the donor mesh's nodes (as Euler2D::outPutCp
+ * averages cells to nodes, Euler2D.cpp:396-422), the nodal donor stencil
+ * (MeshBlock::getInterpolatedSolution, UniTioga/dataUpdate.cpp:6-113) and
+ * the receptor cell as the mean of its vertices (Orchestra.cpp:269-292).
+ * Cells are numbered over the blocks in order (block b's cell c is
+ * sum_{i<b} nc_i + c).  Outputs (host, caller allocated; pass recv NULL to
+ * get the counts): blank[total cells] -- 1 field, 0 hole, -1 receptor (the
+ * cell iblank of getCellIblanks; a receptor cell that has a vertex without
+ * a donor, or donor nodes that touch no field cell, is left a field cell) --
+ * nrecv receptors recv[nrecv], offsets off[nrecv+1], and ndonor donor cells
+ * donor[ndonor] with weights w[ndonor] (ascending donor cell per receptor).
+ * meshes are the blocks given to usg_overset_create.  Host only. */
+int usg_overset_cell_stencil(usg_overset *ov, const usg_mesh_desc *meshes, int nblocks, int *blank, int *nrecv,
+                             int *recv, int *off, int *ndonor, int *donor, double *w);
+
+/* Overset coupling of a context (Euler or laminar flow; host arrays over
+ * its cells, as usg_overset_cell_stencil returns them): every LU-SGS
+ * iteration first sets each receptor cell's state from its donors,
+ * U_r = sum_i w_i U_donor(i) and W_r = cons(U_r); the sweeps then update
+ * field cells only (hole and receptor cells keep their state and take no
+ * increment, so a field cell sees dW = 0 across the overset boundary), and
+ * the residual norm sums field cells only.  Receptors must be distinct
+ * cells blanked -1 and donors cells blanked 1.  blank NULL removes the
+ * coupling.  The reference registers node arrays with tioga and calls
+ * dataUpdate from its driver (Orchestra.cpp:233-292); this is that step
+ * inside the iteration. */
+int usg_set_overset(usg_ctx *ctx, const int *blank, int nrecv, const int *recv, const int *off, const int *donor,
+                    const double *w);
+
+/* Pseudo-time CFL number of the LU-SGS iterations from the next iteration
+ * on: the CFL argument of SolveTime (Euler2D.h:67-77), which the reference's
+ * LUSGSIntegrator passes as 400 (LUSGS_Integrator.h:18).  A schedule that
+ * starts low and grows lets stiff starts (wall spacing of y+ ~ 1) keep a
+ * positive pressure. */
+int usg_set_cfl(usg_ctx *ctx, double cfl);
+
+/* Spatial order of the face reconstruction from the next residual on: 2 is
+ * the reference's limited linear reconstruction (HLLCFluxStrategy.h:35-100,
+ * the default), 1 uses the cell values (limiter phi = 0).  No counterpart
+ * in the reference.  On meshes with y+ ~ 1 wall spacing the second-order
+ * scheme diverges at any CFL number (Euler, laminar and SST alike); the
+ * first-order one converges. */
+int usg_set_order(usg_ctx *ctx, int order);
+
+/* Dual time stepping for unsteady flow (Euler and laminar; no counterpart
+ * in the reference, whose iterations are pseudo-time only): with physical
+ * step dt_phys > 0, each LU-SGS iteration solves the residual plus the
+ * physical-time derivative of W by backward differences,
+ *   R + V (c0 W + c1 W^n + c2 W^(n-1)),   D + c0 V,
+ * with (c0, c1, c2) = (1, -1, 0) / dt_phys on the first physical step and
+ * (3/2, -2, 1/2) / dt_phys after it when order is 2.  usg_dual_time_advance
+ * starts a physical step (the current state becomes W^n, the previous W^n
+ * becomes W^(n-1)); LU-SGS iterations are then its sub-iterations.  The
+ * term is off until the first advance; dt_phys <= 0 switches it off. */
+int usg_dual_time(usg_ctx *ctx, double dt_phys, int order);
+int usg_dual_time_advance(usg_ctx *ctx);
 
 /* The reference's "Cp_<step>" text file (Tecplot POINT zone of x and C_p,
  * the first point repeated at the end) of a curve from usg_surface_cp. */

@@ -27,7 +27,7 @@ class _Mesh(C.Structure):
                 ("xy", _dp), ("fnode", _ip), ("flc", _ip), ("frc", _ip),
                 ("cnode", _ip), ("cnv", _ip), ("cface", _ip), ("cnf", _ip),
                 ("vol", _dp), ("cc", _dp), ("fn", _dp), ("fa", _dp), ("fc", _dp),
-                ("wdist", _dp)]
+                ("wdist", _dp), ("wdist_wall", _dp)]
 
 
 _SOLVER_DOUBLES_A = ["R_gas", "Gamma", "PI", "fs_mach", "fs_AOA", "fs_density", "fs_pressure",
@@ -52,7 +52,7 @@ class _Solver(C.Structure):
                 + [("fs_turb", C.c_double * 2), ("fsnd_turb", C.c_double * 2)]
                 + [("hllc_pstar_violations", C.c_long), ("Fv_face", _dp), ("Fv_all", _dp)]
                 + [("dt_phys", C.c_double), ("dt_order", C.c_int), ("dt_steps", C.c_int),
-                   ("dts", C.c_double * 3), ("Wt1", _dp), ("Wt2", _dp), ("blank", _ip)])
+                   ("dts", C.c_double * 3), ("Wt1", _dp), ("Wt2", _dp), ("blank", _ip), ("order", C.c_int)])
 
 
 def build(force: bool = False) -> str:
@@ -92,6 +92,7 @@ def lib():
         L.orc_dual_time_advance.argtypes = [C.POINTER(_Solver)]
         L.orc_set_blank.argtypes = [C.POINTER(_Solver), _ip]
         L.orc_set_cells.argtypes = [C.POINTER(_Solver), C.c_int, _ip, _dp]
+        L.orc_set_order.argtypes = [C.POINTER(_Solver), C.c_int]
         L.orc_hllc_face.argtypes = [_dp, _dp, _dp, C.c_double, _dp, _ip]
         L.orc_lusgs_sweeps.argtypes = [C.POINTER(_Solver)]
         L.orc_cell_volume.restype = C.c_double
@@ -127,6 +128,7 @@ class OracleMesh:
         self.fa = _arr(m.fa, m.nf)
         self.fc = _arr(m.fc, 2 * m.nf).reshape(m.nf, 2)
         self.wdist = _arr(m.wdist, m.nc)
+        self.wdist_wall = _arr(m.wdist_wall, m.nc)
         self.cnode = np.ctypeslib.as_array(m.cnode, shape=(4 * m.nc,)).reshape(m.nc, 4)
         self.cnv = np.ctypeslib.as_array(m.cnv, shape=(m.nc,))
         self.cface = np.ctypeslib.as_array(m.cface, shape=(4 * m.nc,)).reshape(m.nc, 4)
@@ -169,6 +171,10 @@ class OracleSolver:
     def R(self):
         return self.field("R", 4)
 
+    def init_data(self):
+        """Free-stream cold start (orc_init_data)."""
+        lib().orc_init_data(self.ptr)
+
     def set_prim(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)
         lib().orc_set_prim(self.ptr, U.ctypes.data_as(_dp))
@@ -202,6 +208,9 @@ class OracleSolver:
             return
         b = np.ascontiguousarray(blank, np.int32)
         lib().orc_set_blank(self.ptr, b.ctypes.data_as(_ip))
+
+    def set_order(self, order: int):
+        lib().orc_set_order(self.ptr, int(order))
 
     def set_cells(self, idx, U):
         """Primitive states U[n, 4] of cells idx (W from U), e.g. overset receptors."""

@@ -155,6 +155,18 @@ double orc_cell_volume(const double *xy, int nv) {
   return cell_volume_2d(xv, nv);
 }
 
+/* Distance from (cx, cy) to the segment p0-p1: the foot of the
+ * perpendicular clamped to the segment. */
+static double segment_distance(double cx, double cy, const double *p0, const double *p1) {
+  double ax = p1[0] - p0[0], ay = p1[1] - p0[1];
+  double px = cx - p0[0], py = cy - p0[1];
+  double t = (px * ax + py * ay) / (ax * ax + ay * ay);
+  if (!(t > 0.0)) t = 0.0;
+  if (t > 1.0) t = 1.0;
+  double qx = px - t * ax, qy = py - t * ay;
+  return sqrt(qx * qx + qy * qy);
+}
+
 orc_mesh *orc_mesh_create(int nn, const double *xy, int nf, const int *fnode,
                           const int *flc, const int *frc, int nc) {
   orc_mesh *m = (orc_mesh *)xcalloc(1, sizeof(orc_mesh));
@@ -255,6 +267,21 @@ orc_mesh *orc_mesh_create(int nn, const double *xy, int nf, const int *fnode,
     }
     m->wdist[i] = d;
   }
+  /* The SST model's wall distance (deviation D6): the Euclidean distance
+   * from the cell centre to the nearest wall face segment.  The reference's
+   * projected distance above is |(x_c - x_f) . n_f|, which is small for any
+   * point near the tangent line of some wall face however far it is from
+   * the wall; its SST (never executed there) would blend to the near-wall
+   * branch in most of the far field. */
+  m->wdist_wall = (double *)xcalloc(nc, sizeof(double));
+  for (int i = 0; i < nc; i++) {
+    double d = 100000000000.0;
+    for (int w = 0; w < nw; w++) {
+      int k = wf[w];
+      d = smin(d, segment_distance(m->cc[2 * i], m->cc[2 * i + 1], &xy[2 * fnode[2 * k]], &xy[2 * fnode[2 * k + 1]]));
+    }
+    m->wdist_wall[i] = d;
+  }
   free(wf);
   return m;
 }
@@ -263,7 +290,7 @@ void orc_mesh_destroy(orc_mesh *m) {
   if (!m) return;
   free(m->xy); free(m->fnode); free(m->flc); free(m->frc);
   free(m->cnode); free(m->cnv); free(m->cface); free(m->cnf);
-  free(m->vol); free(m->cc); free(m->fn); free(m->fa); free(m->fc); free(m->wdist);
+  free(m->vol); free(m->cc); free(m->fn); free(m->fa); free(m->fc); free(m->wdist); free(m->wdist_wall);
   free(m);
 }
 
@@ -327,6 +354,7 @@ orc_solver *orc_solver_create(orc_mesh *m, int flow, double ma, double aoa_deg, 
   /* LUSGS_Integrator.h:18 ; RungeKuttaStrategy.h:122-131 */
   s->cfl_lusgs = 400;
   s->cfl_rk = 0.8;
+  s->order = 2;
   s->rk_alpha[0] = 1.0 / 4;
   s->rk_alpha[1] = 1.0 / 6;
   s->rk_alpha[2] = 3.0 / 8;
@@ -669,6 +697,8 @@ static void compute_limiter(orc_solver *s) {
 /* HLLCFluxStrategy::updateLeftRight (HLLCFluxStrategy.h:35-100) */
 static void update_left_right(orc_solver *s) {
   compute_limiter(s);
+  if (s->order == 1)
+    for (int k = 0; k < s->m->nc * NE; k++) s->phi[k] = 0.0;
   const orc_mesh *m = s->m;
   for (int k = 0; k < m->nf; k++) {
     int lc = m->flc[k], rc = m->frc[k];
@@ -872,7 +902,7 @@ static void sst_face_values(orc_solver *s) {
       s->tUedge[2 * k + 1] = 0.5 * (s->tU[2 * lC + 1] + s->tU[2 * rC + 1]);
     } else if (rC == ORC_WALL) {
       s->tUedge[2 * k] = 0.0;
-      double dis = m->wdist[lC];
+      double dis = m->wdist_wall[lC];
       s->tUedge[2 * k + 1] = 10.0 * 6.0 * s->mu_edge[k] / (s->U_edge[NE * k] * sst_beta1 * dis * dis);
     } else {
       double Vn = dot2(s->U_edge[NE * k + 2], s->U_edge[NE * k + 3], m->fn[2 * k], m->fn[2 * k + 1]);
@@ -917,7 +947,7 @@ static void sst_f1f2(orc_solver *s) {
     const double *gk = &s->tgrad[4 * k], *gw = &s->tgrad[4 * k + 2];
     double gradVal = dot2(gk[0], gk[1], gw[0], gw[1]);
     double rho = s->U[NE * k];
-    double dis = m->wdist[k];
+    double dis = m->wdist_wall[k];
     double K = s->tU[2 * k], w = s->tU[2 * k + 1];
     double muL = s->mu[k];
     double CD_kw = smax(2.0 * rho * sst_sigma_w2 / w * gradVal, 1.0e-20);
@@ -1097,6 +1127,8 @@ void orc_set_blank(orc_solver *s, const int *blank) {
   s->blank = (int *)xcalloc(s->m->nc, sizeof(int));
   memcpy(s->blank, blank, sizeof(int) * s->m->nc);
 }
+
+void orc_set_order(orc_solver *s, int order) { s->order = order; }
 
 void orc_set_cells(orc_solver *s, int n, const int *idx, const double *U) {
   for (int i = 0; i < n; i++) {

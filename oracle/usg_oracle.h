@@ -40,6 +40,7 @@ typedef struct orc_mesh {
   double *fa;        /* [nf]   face length                               */
   double *fc;        /* [nf*2] face midpoint                             */
   double *wdist;     /* [nc]   projected wall distance                   */
+  double *wdist_wall; /* [nc]  distance to the nearest wall face segment (SST) */
 } orc_mesh;
 
 typedef struct orc_solver {
@@ -82,6 +83,8 @@ typedef struct orc_solver {
   /* overset cell blanking (NULL: every cell a field cell): 1 field, 0 hole,
    * -1 receptor; only field cells are updated and enter the norm */
   int *blank;
+  /* reconstruction order: 2 (the reference's limited linear) or 1 (cell values) */
+  int order;
 } orc_solver;
 
 /* mesh / geometry (edge3d_int.h, cell3d_int.h, cellVolume.cpp, CellBrick.cpp) */
@@ -107,6 +110,8 @@ void orc_dual_time_advance(orc_solver *s);
 /* overset: cell blanking (NULL clears) and receptor states U[n*4] of cells idx[n] */
 void orc_set_blank(orc_solver *s, const int *blank);
 void orc_set_cells(orc_solver *s, int n, const int *idx, const double *U);
+/* reconstruction order (no counterpart in the reference, which is second order) */
+void orc_set_order(orc_solver *s, int order);
 /* standalone pieces for invariant tests */
 void orc_lusgs_sweeps(orc_solver *s);
 double orc_cell_volume(const double *xy, int nv);

@@ -23,7 +23,7 @@ class FakeSolver:
 def test_key_rounds_shape(monkeypatch):
     monkeypatch.setattr(autotune, "device_arch", lambda: "gfx950")
     k = autotune.shape_key(FakeSolver(), 0, 1516320)
-    assert k == "gfx950|2|0|1|1000000|1.5"
+    assert k == "gfx950|fused-tiles|2|0|1|1000000|1.5"
     assert autotune._round2(23456) == 23000 and autotune._round2(7) == 7
 
 
@@ -33,7 +33,7 @@ def test_miss_searches_then_hit_applies(tmp_path, monkeypatch):
     s = FakeSolver()
     blocks, hit = autotune.tune(s, 0, 1516320, cache=cache)
     assert not hit and s.searched == 1 and blocks == [128, 256, 64, 128]
-    assert json.load(open(cache)) == {"gfx950|2|0|1|1000000|1.5": [128, 256, 64, 128]}
+    assert json.load(open(cache)) == {"gfx950|fused-tiles|2|0|1|1000000|1.5": [128, 256, 64, 128]}
     s2 = FakeSolver()
     blocks, hit = autotune.tune(s2, 0, 1516320, cache=cache)
     assert hit and s2.searched == 0 and s2.applied == [128, 256, 64, 128]

@@ -192,3 +192,70 @@ def test_graph_replays_match_eager(flow):
         g.residual()          # eager work between replays
         e.residual()
         assert np.array_equal(g.R, e.R)
+
+
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", CASES, ids=[c[0] for c in CASES])
+def test_fused_face_cell_equals_two_launches(name, mk, flow, mach, aoa, re):
+    """The fused face-cell launch (face record in LDS per tile) gives the
+    two-launch path's residual fields and LU-SGS trajectory bit for bit."""
+    m = mk()
+    a = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re)
+    b = usg.Solver(m, flow=flow, mach=mach, aoa=aoa, reynolds=re)
+    b.fused(False)
+    U = perturbed(m, usg.mesh_geometry(m)["cc"], a.freestream(), seed=4)
+    a.set_prim(U)
+    b.set_prim(U)
+    a.residual()
+    b.residual()
+    fields = [usg.F_RES, usg.F_LAMC, usg.F_DT] + ([usg.F_LAMV] if flow != usg.EULER else []) + \
+             ([usg.F_TRES] if flow == usg.SST else [])
+    for fid in fields:
+        assert np.array_equal(a.field(fid), b.field(fid)), fid
+    ha = a.lusgs(6, history=True)
+    hb = b.lusgs(6, history=True)
+    assert np.array_equal(ha, hb) and np.array_equal(a.U, b.U)
+    assert a.forces() == b.forces()
+
+
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_face_states_match_oracle(name, mk, flow, mach, aoa, re):
+    """The U_edge getter (Euler2D::updateEdgeValues' face states) of a
+    perturbed state equals the oracle's U_edge bit for bit (libm pow in the
+    far-field state aside)."""
+    m = mk()
+    orc, gpu, om = _pair(m, flow, mach, aoa, re)
+    orc.residual()
+    e = np.max(relerr(gpu.field(usg.F_UEDGE), orc.field("U_edge", 4, "nf")))
+    assert e <= TOL_RES, e
+
+
+def test_sst_wall_distance_matches_oracle():
+    """The SST model's wall distance (the distance to the nearest wall face
+    segment, computed on the GPU at context creation) equals the oracle's
+    bit for bit, on a single mesh and inside a batch of two."""
+    m = airfoil_small()
+    om = O.OracleMesh(m)
+    gpu = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
+    assert np.array_equal(gpu.field(usg.F_WDIST), om.wdist_wall)
+    p2 = part2()
+    b = usg.BatchSolver([p2, m], flow=usg.SST, mach=0.15, aoa=[0.0, 4.0], reynolds=6e6)
+    w = b.field(usg.F_WDIST)
+    assert np.array_equal(w[b.case_slice(1)], om.wdist_wall)
+    op2 = O.OracleMesh(p2)
+    assert np.array_equal(w[b.case_slice(0)], op2.wdist_wall)
+
+
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", [CASES[1], CASES[3]], ids=["bump", "airfoil-sst"])
+def test_first_order_trajectory(name, mk, flow, mach, aoa, re):
+    """First-order reconstruction (usg_set_order(1)): 20 LU-SGS iterations
+    against the oracle with the same order."""
+    m = mk()
+    orc, gpu, om = _pair(m, flow, mach, aoa, re, perturb=False)
+    orc.set_order(1)
+    gpu.set_order(1)
+    for _ in range(20):
+        orc.lusgs_step()
+    gpu.lusgs(20)
+    e = np.max(relerr(gpu.U, orc.U))
+    print(f"{name} first order: U rel {e:.3e}")
+    assert e <= TOL_TRAJ

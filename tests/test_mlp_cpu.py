@@ -73,3 +73,15 @@ def test_mlp_exports_and_shape_checks():
         except mlp.MlpError:
             continue
         raise AssertionError(bad)
+
+
+def test_warm_start_screen():
+    """The judge's screen of a predicted field (tmp_amd.warmstart):
+    finite, positive density and pressure everywhere."""
+    from tmp_amd import warmstart
+    U = np.tile([1.0, 1.0, 0.3, 0.0], (10, 1))
+    assert warmstart.screen_prediction(U)
+    for j, v in ((0, -1e-3), (1, 0.0), (2, np.nan), (3, np.inf)):
+        bad = U.copy()
+        bad[4, j] = v
+        assert not warmstart.screen_prediction(bad)

@@ -44,7 +44,8 @@ def test_geometry_closure_and_area():
     assert np.abs(acc).max() < 1e-14
     assert abs(om.vol.sum() - 2.0) < 1e-14
     t = mesh.rect(6, 4, jitter=0.2, seed=4, triangles=True)
-    assert abs(O.OracleMesh(t).vol.sum() - 1.0) < 1e-14
+    ot = O.OracleMesh(t)
+    assert abs(ot.vol.sum() - 1.0) < 1e-14
 
 
 def test_wall_distance_flat_channel():
@@ -383,3 +384,37 @@ def test_limiter_epsilon_uses_left_volume_to_the_first_power():
     for alt in (125.0 * vol[1], 125.0 * vol[0] ** 1.5):
         assert abs(alt / (2 * d2 * d2 + alt) - expect) > 0.05
     assert np.all(phi[1, 1:] == 1.0)           # p, u, v uniform: unlimited
+
+
+def test_first_order_reconstruction():
+    """With order 1 every interior face's reconstructed states are the two
+    cell values exactly (phi = 0), so the residual is the first-order
+    Godunov-type sum of HLLC fluxes of cell states: checked face by face
+    against the oracle's own HLLC of (U_L, U_R) (catches a partial phi
+    reset or a reconstruction that still adds the gradient term)."""
+    m = bump()
+    om = O.OracleMesh(m)
+    s = O.OracleSolver(om, O.EULER, 0.5, 0.0, 1e6)
+    s.set_prim(perturbed(m, om.cc, list(s.s.fs_primVar)[:4], amp=0.05, seed=8))
+    s.set_order(1)
+    s.residual()
+    UL, UR, U = s.field("UL", 4, "nf"), s.field("UR", 4, "nf"), s.U
+    inner = np.nonzero(m.frc >= 0)[0]
+    assert np.array_equal(UL[inner], U[m.flc[inner]]) and np.array_equal(UR[inner], U[m.frc[inner]])
+    R1 = s.R.copy()
+    R = np.zeros_like(R1)
+    Ue = s.field("U_edge", 4, "nf")
+    for f in range(m.nf):
+        l, r, n, A = m.flc[f], m.frc[f], om.fn[f], om.fa[f]
+        if r >= 0:
+            F, _ = O.hllc_face(U[l], U[r], n, A)
+            R[l] += F
+            R[r] -= F
+        elif r == mesh.WALL:
+            R[l] += np.array([0.0, 0.0, Ue[f, 1] * n[0], Ue[f, 1] * n[1]]) * A
+        else:
+            R[l] += _euler_flux(Ue[f], n) * A
+    assert np.allclose(R1, R, rtol=1e-12, atol=1e-14)
+    s.set_order(2)
+    s.residual()
+    assert np.max(np.abs(s.R - R1)) > 1e-6

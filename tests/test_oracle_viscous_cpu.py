@@ -250,7 +250,7 @@ def test_sst_blending_functions_closed_form():
     k0, w0 = 0.01, 10.0
     target = rng.uniform(0.2, 2.0, m.nc)
     d = np.sqrt(k0) / (BSTAR * w0 * target)
-    om.wdist[:] = d
+    om.wdist_wall[:] = d
     _set_sst_state(s, U, k0, w0)
     s.residual()
     assert np.all(500 * nu / (d * d * w0) < 0.5 * target)
@@ -260,7 +260,7 @@ def test_sst_blending_functions_closed_form():
     # (2) viscous branch: k tiny, d chosen for 500 nu/(d^2 omega) in [0.2, 2]
     k0, w0 = 1e-14, 10.0
     d = np.sqrt(500 * nu / (w0 * target))
-    om.wdist[:] = d
+    om.wdist_wall[:] = d
     _set_sst_state(s, U, k0, w0)
     s.residual()
     assert np.all(np.sqrt(k0) / (BSTAR * w0 * d) < 0.5 * target)
@@ -273,7 +273,7 @@ def test_sst_blending_functions_closed_form():
     cap_target = rng.uniform(0.3, 1.2, m.nc)
     CD = 2 * rho * SIGW2 * kx * wx / wc
     d = np.sqrt(4 * rho * SIGW2 * kc / (CD * cap_target))
-    om.wdist[:] = d
+    om.wdist_wall[:] = d
     _set_sst_state(s, U, kc, wc)
     s.residual()
     assert np.allclose(s.field("tgrad", 4)[inner], np.tile([kx, 0, wx, 0], (inner.sum(), 1)), rtol=1e-12,
@@ -304,7 +304,7 @@ def test_sst_eddy_viscosity_closed_form():
     U[:, 2] = a * y
     U[:, 3] = 0.0
     t = np.random.default_rng(5).uniform(0.6, 1.2, m.nc)
-    om.wdist[:] = 2 * np.sqrt(k0) / (BSTAR * w0 * t)   # arg2 = t: F2 = tanh(t^2) in [0.35, 0.89]
+    om.wdist_wall[:] = 2 * np.sqrt(k0) / (BSTAR * w0 * t)   # arg2 = t: F2 = tanh(t^2) in [0.35, 0.89]
     _set_sst_state(s, U, k0, w0)
     s.residual()
     F2 = np.tanh(t * t)
@@ -340,7 +340,7 @@ def test_sst_production_and_sources_closed_form():
     fs = list(s.s.fs_primVar)[:4]
     rho, k0, w0 = 1.0, 0.05, 4.0
     far = 1e11                                     # no wall: F1 = 0, beta2 and C_w2
-    om.wdist[:] = far
+    om.wdist_wall[:] = far
 
     def run(u, v, k, w, mut):
         U = np.column_stack([np.full(m.nc, rho), np.full(m.nc, rho), u, v])
@@ -378,7 +378,7 @@ def test_sst_production_and_sources_closed_form():
     kx, wx = 0.04, 3.0
     kc, wc = k0 + kx * x, w0 + wx * x
     near = x > 0.5
-    om.wdist[:] = np.where(near, 1e-7, far)
+    om.wdist_wall[:] = np.where(near, 1e-7, far)
     tR, F1 = run(np.zeros(m.nc), np.zeros(m.nc), kc, wc, 1e-3)
     assert np.all(F1[near] == 1.0) and np.all(F1[~near] < 1e-30)
     # cells whose neighbours share their F1 (the diffusion coefficient is uniform around them)
@@ -407,7 +407,7 @@ def test_sst_wall_omega():
     Te = s.field("tUedge", 2, "nf")
     mu = np.sqrt(G) * mach / re                  # free-stream T at the no-slip wall
     walls = np.nonzero(m.frc == mesh.WALL)[0]
-    d = om.wdist[m.flc[walls]]
+    d = om.wdist_wall[m.flc[walls]]
     assert np.all(Te[walls, 0] == 0.0)
     assert np.allclose(Te[walls, 1], 60.0 * mu / (1.0 * BETA1 * d * d), rtol=1e-13, atol=0)
 
@@ -485,3 +485,28 @@ def test_viscous_force_subtracts_the_wall_shear():
     mu = np.sqrt(G) * mach / re
     assert abs(fx - mu * a * 4.0) < 1e-12 * mu * a * 4.0
     assert abs(fy - (-1.0 * 4.0)) < 1e-12      # p n A of p = 1 on the wall, normal (0, -1)
+
+
+def test_sst_wall_distance_closed_form():
+    """The SST model's wall distance (deviation D6) is the Euclidean
+    distance to the nearest wall face segment.  On the cylinder O-grid every
+    cell centre (the vertex mean) lies on the bisector of one wall edge of
+    the inscribed N-gon, at radius (r_j + r_j+1)/2 cos(pi/N), so its distance
+    is ((r_j + r_j+1)/2 - R) cos(pi/N); in a flat channel it is the height.
+    The reference's projected distance |(x_c - x_f) . n_f| (kept as the
+    geometry's wdist) is far smaller than the true distance away from the
+    wall, which this pins too."""
+    ni, nj, dr1, ratio, dmax = 64, 12, 0.01, 1.3, 0.5
+    og, _ = mesh.cylinder_ogrid(ni=ni, nj=nj, dr1=dr1, ratio=ratio, dr_max=dmax)
+    om = O.OracleMesh(og)
+    r = mesh.ogrid_radii(nj, 0.5, dr1, ratio, dmax)
+    rc = np.hypot(om.cc[:, 0], om.cc[:, 1]) / np.cos(np.pi / ni)    # (r_j + r_j+1) / 2
+    expect = (rc - 0.5) * np.cos(np.pi / ni)
+    assert np.allclose(om.wdist_wall, expect, rtol=1e-12, atol=1e-15)
+    assert np.all(np.min(np.abs(rc[:, None] - 0.5 * (r[:-1] + r[1:])[None, :]), axis=1) < 1e-12)
+    ch = mesh.channel_bump(ni=30, nj=12, nq=3, seed=5, height=0.0)
+    oc = O.OracleMesh(ch)
+    y = oc.cc[:, 1]
+    assert np.allclose(oc.wdist_wall, np.minimum(y, 1 - y), rtol=0, atol=1e-15)
+    far = expect > 0.2
+    assert np.median(om.wdist[far] / om.wdist_wall[far]) < 0.2

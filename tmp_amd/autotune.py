@@ -5,7 +5,10 @@
 problem shape: it looks the shape up in a JSON cache and, on a miss, runs
 the library's timed search (``usg_autotune``: coordinate descent over 64,
 128 and 256 threads, the state restored afterwards) and stores the result.
-The key is (GPU arch, flow type, precision, number of cases, cells rounded
+In FP64 the face fluxes and cell gathers run as one fused launch whose
+tiles are the cell group's blocks, so the cell group's size is the face
+tile: 64, 128 or 256 cells and every face they touch.
+The key is (GPU arch, the launch scheme, flow type, precision, number of cases, cells rounded
 to 2 significant digits, faces/cells ratio rounded to 0.1).
 """
 from __future__ import annotations
@@ -34,7 +37,7 @@ def _round2(n: int) -> int:
 
 
 def shape_key(solver, precision: int, nf: int) -> str:
-    return "|".join([device_arch(), str(solver.flow), str(precision), str(solver.ncase),
+    return "|".join([device_arch(), "fused-tiles", str(solver.flow), str(precision), str(solver.ncase),
                      str(_round2(solver.nc)), f"{nf / solver.nc:.1f}"])
 
 

@@ -6,7 +6,7 @@ import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libusg.so")
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("solver.hip", "mlp.hip", "geometry.cpp", "stencil.cpp", "output.cpp", "overset.cpp", "renumber.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("solver.hip", "mlp.hip", "geometry.cpp", "stencil.cpp", "output.cpp", "overset.cpp", "renumber.cpp", "coupling.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", f) for f in ("kernels.h", "geometry.h")] + [
     os.path.join(os.path.dirname(HERE), "include", h) for h in ("usg.h", "usg_mlp.h")]
 

@@ -60,7 +60,13 @@ struct DevMesh {
   // bit 0: the cell has no lower neighbour (forward wave 0), bit 1: no
   // upper neighbour (backward wave 0); those waves are fused into others
   const unsigned char *wflag;
+  // overset cell blanking (null: every cell is a field cell): 1 field,
+  // 0 hole, -1 receptor.  Only field cells are updated by the sweeps and
+  // enter the residual norm; receptors are set from their donors.
+  const signed char *blank;
 };
+
+__device__ __forceinline__ bool frozen_cell(const DevMesh &m, int c) { return m.blank && m.blank[c] != 1; }
 
 // State, structure-of-arrays ([var*nc + cell]).
 struct DevState {
@@ -88,6 +94,9 @@ struct DevState {
   double *F1, *F2, *mut, *mutn; // [nc]
   double *tR, *tdW1, *tdW;      // [2][nc]
   int *err;                     // [2] invariant violation counters
+  // dual time stepping: the conservative state at the two previous
+  // physical time levels, W^n and W^(n-1) ([4][nc]; null when off)
+  double *Wt1, *Wt2;
   // FP32 storage of the mixed-precision mode (null in FP64 mode, where the
   // double arrays above hold these quantities)
   float *gradf, *phif, *gradTf, *tgradf, *F1f, *F2f, *FFf;
@@ -119,6 +128,13 @@ struct Consts {
   double fsnd_turb[2]; // free-stream (k, omega)
   double fsnd_mut;     // free-stream eddy viscosity
   double sqrt_gamma;   // sqrt(1.4)
+  // dual time stepping: backward-difference coefficients over the physical
+  // step, the residual gains V (dts[0] W + dts[1] W^n + dts[2] W^(n-1)) and
+  // the diagonal dts[0] V; all zero when off
+  double dts[3];
+  // spatial order of the face reconstruction: 2 (the reference's limited
+  // linear reconstruction) or 1 (the cell values, phi = 0)
+  int order;
 };
 
 // Constants of the case cell c belongs to: the kernel argument for a single

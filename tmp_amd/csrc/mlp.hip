@@ -756,10 +756,12 @@ int usg_mlp_gradients(usg_mlp *m, const float *x, const float *y, int n, float *
 
 int usg_mlp_set_data(usg_mlp *m, const float *x, const float *y, long n) {
   if (!m || !x || !y || n <= 0) return fail(-1, "bad argument");
-  if (m->xs) {
+  // the old set is gone from here on: a failure below leaves no data set
+  m->nset = 0;
+  if (m->xs || m->ys) {
     HIPCHK(hipStreamSynchronize(m->s));
-    hipFree(m->xs);
-    hipFree(m->ys);
+    if (m->xs) hipFree(m->xs);
+    if (m->ys) hipFree(m->ys);
     m->xs = nullptr;
     m->ys = nullptr;
   }
@@ -822,24 +824,29 @@ int usg_mlp_train_epoch(usg_mlp *m, const int *perm, long n, int batch, float lr
 
 int usg_mlp_mfma_check(const float *a, const float *b, float *d) {
   if (!a || !b || !d) return fail(-1, "null argument");
-  bf16 *da = nullptr, *db = nullptr;
-  float *dd = nullptr;
+  // device buffers released on every return
+  struct Bufs {
+    void *p[3] = {nullptr, nullptr, nullptr};
+    ~Bufs() {
+      for (void *q : p)
+        if (q) (void)hipFree(q);
+    }
+  } bufs;
   std::vector<bf16> ha(16 * 32), hb(32 * 16);
   for (int i = 0; i < 16 * 32; i++) {
     ha[i] = to_bf16(a[i]);
     hb[i] = to_bf16(b[i]);
   }
-  HIPCHK(hipMalloc(&da, ha.size() * 2));
-  HIPCHK(hipMalloc(&db, hb.size() * 2));
-  HIPCHK(hipMalloc(&dd, 256 * 4));
+  HIPCHK(hipMalloc(&bufs.p[0], ha.size() * 2));
+  HIPCHK(hipMalloc(&bufs.p[1], hb.size() * 2));
+  HIPCHK(hipMalloc(&bufs.p[2], 256 * 4));
+  bf16 *da = (bf16 *)bufs.p[0], *db = (bf16 *)bufs.p[1];
+  float *dd = (float *)bufs.p[2];
   HIPCHK(hipMemcpy(da, ha.data(), ha.size() * 2, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(db, hb.data(), hb.size() * 2, hipMemcpyHostToDevice));
   hipLaunchKernelGGL(k_mfma_check, dim3(1), dim3(64), 0, 0, da, db, dd);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpy(d, dd, 256 * 4, hipMemcpyDeviceToHost));
-  hipFree(da);
-  hipFree(db);
-  hipFree(dd);
   return 0;
 }
 

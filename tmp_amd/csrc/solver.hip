@@ -290,6 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
       ph[j] = smin(ph[j], venkat(d1max[j], d1min[j], delta2, eps2));
     }
   }
+  if (k.order == 1)
+    for (int j = 0; j < 4; j++) ph[j] = 0.0;  // first order: the face states are the cell values
   T *P = pick<T>(s.phi, s.phif);
   for (int j = 0; j < 4; j++) P[j * nc + c] = ph[j];
 
@@ -350,23 +352,22 @@ __device__ __forceinline__ double lusgs_ra(const double *Ua, double n0, double n
   return rA;
 }
 
-// k_face: every flux of one face, computed once.  HLLCFluxStrategy::computeFlux
+// face_terms: every flux of one face, computed once.  HLLCFluxStrategy::computeFlux
 // (HLLCFluxStrategy.cpp:4-171) with the limited reconstruction of
 // updateLeftRight (HLLCFluxStrategy.h:35-100); the viscous face flux of
 // LaminarFluxStrategy::computeFlux (LaminarFluxStrategy.h:110-216); the SST
 // convective minus diffusive face flux (SSTKWStrategy.h:616-659,783-920).
-// Outputs per face: Fc[4], Fv[4] (viscous), Ft[2] (SST), all area-weighted,
-// oriented out of the left cell.
+// Outputs per face, through out(q, value): q 0-3 the convective flux, 4 the
+// spectral term, 5-8 the viscous flux, 9 the viscous spectral term, 10-11
+// the SST flux, all area-weighted and oriented out of the left cell; the
+// sweeps' r_A of both sides go to their cell slots.
 // The single-case FP64 variants are held to 96 VGPRs (5 waves per SIMD
 // instead of 4; they fit without spilling and run ~2% faster on the 1 M-cell
 // SST mesh); the batched FP64 variants would spill and keep the default.
 // ---------------------------------------------------------------------------
-template <int FLOW, typename T, bool B>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value && !B ? 5 : 1)))
-void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__restrict__ items, const XSlice sl) {
+template <int FLOW, typename T, bool B, typename Out>
+__device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, const Consts &k_, int f, Out out) {
   using F = Flow<FLOW>;
-  int f;
-  if (!xcd_item(items, sl, f)) return;
   const int nc = m.nc, nf = m.nf;
   const T Gamma = T(USG_GAMMA);
   const int L = m.flc[f], Rr = m.frc[f];
@@ -397,11 +398,9 @@ void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__res
       Fx[j] = T(Fxd[j]);
     }
   }
-  // face record, quantity q at FF[q*nf + f] (one array per quantity, so a
-  // wave's store of one quantity is contiguous): flux 0-3, spectral term 4,
-  // viscous flux 5-8, viscous spectral term 9, SST 10-11
-  T *const FF = pick<T>(s.FF, s.FFf) + f;
-  for (int j = 0; j < 4; j++) FF[(size_t)j * nf] = Fx[j];
+  // face record: flux 0-3, spectral term 4, viscous flux 5-8, viscous
+  // spectral term 9, SST 10-11
+  for (int j = 0; j < 4; j++) out(j, Fx[j]);
   if (inner) {
     // r_A of the sweeps' off-diagonal terms for each side, from the cell
     // states they would otherwise gather: the left cell receives from the
@@ -421,7 +420,7 @@ void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__res
     // spectral radius term (Euler2D::solveSpectralRadius, Euler2D.cpp:242-266)
     T cs = sqrt(Gamma * Ue[1] / Ue[0]);
     T Vn = dot2(Ue[2], Ue[3], n0, n1);
-    FF[4 * (size_t)nf] = (fabs(Vn) + cs) * A;
+    out(4, (fabs(Vn) + cs) * A);
   }
   if (!F::visc) return;
   const T *G = pickc<T>(s.grad, s.gradf);
@@ -491,10 +490,10 @@ void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__res
   Fv[1] = ((Ue[2] * taoxx + Ue[3] * taoxy - qx) * n0 + (Ue[2] * taoxy + Ue[3] * taoyy - qy) * n1) * A;
   Fv[2] = (taoxx * n0 + taoxy * n1) * A;
   Fv[3] = (taoxy * n0 + taoyy * n1) * A;
-  for (int j = 0; j < 4; j++) FF[(size_t)(5 + j) * nf] = Fv[j];
+  for (int j = 0; j < 4; j++) out(5 + j, Fv[j]);
   // viscous spectral radius term (ViscousFlow2D.cpp:243-249); each cell
   // divides it by its own volume
-  FF[9 * (size_t)nf] = smax(T(4) / (T(3.0) * Ue[0]), Gamma / Ue[0]) * mpr * A * A;
+  out(9, smax(T(4) / (T(3.0) * Ue[0]), Gamma / Ue[0]) * mpr * A * A);
   if (!F::sst) return;
   // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
   const T Vn = dot2(Ue[2], Ue[3], n0, n1);
@@ -557,8 +556,26 @@ void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__res
   T Fv1 = txx_w * n0 + tyy_w * n1;
   Fv0 *= A;
   Fv1 *= A;
-  FF[10 * (size_t)nf] = Fct[0] - Fv0;
-  FF[11 * (size_t)nf] = Fct[1] - Fv1;
+  out(10, Fct[0] - Fv0);
+  out(11, Fct[1] - Fv1);
+}
+
+
+// The face record in global memory, quantity q at FF[q*nf + f] (one array
+// per quantity, so a wave's store of one quantity is contiguous).
+template <typename T>
+struct FaceOutGlobal {
+  T *FF;
+  int nf;
+  __device__ __forceinline__ void operator()(int q, T v) const { FF[(size_t)q * nf] = v; }
+};
+
+template <int FLOW, typename T, bool B>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value && !B ? 5 : 1)))
+void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__restrict__ items, const XSlice sl) {
+  int f;
+  if (!xcd_item(items, sl, f)) return;
+  face_terms<FLOW, T, B>(m, s, k_, f, FaceOutGlobal<T>{pick<T>(s.FF, s.FFf) + f, m.nf});
 }
 
 // ---------------------------------------------------------------------------
@@ -574,15 +591,39 @@ __device__ __forceinline__ void fwd_store(const DevMesh &m, const DevState &s, i
                                           const double *tFi, double diag, double *dW1, double *tdW1) {
   using F = Flow<FLOW>;
   const int nc = m.nc;
+  // overset hole and receptor cells take no increment (their lower
+  // neighbours then see dW1 = 0 across the overset boundary)
+  const bool fz = frozen_cell(m, c);
   for (int j = 0; j < 4; j++) {
-    dW1[j] = (-s.R[j * nc + c] - dFi[j]) / diag;
+    dW1[j] = fz ? 0.0 : (-s.R[j * nc + c] - dFi[j]) / diag;
     s.dW1[j * nc + c] = dW1[j];
   }
   if (F::sst)
     for (int q = 0; q < 2; q++) {
-      tdW1[q] = (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
+      tdW1[q] = fz ? 0.0 : (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
       s.tdW1[q * nc + c] = tdW1[q];
     }
+}
+
+// A cell that is not updated this iteration carries its state into the next
+// buffer unchanged (a frozen batch case, an overset hole or receptor cell).
+template <int FLOW>
+__device__ __forceinline__ void carry_state(const DevMesh &m, const DevState &s, int c) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  for (int j = 0; j < 4; j++) {
+    s.dW[j * nc + c] = 0.0;
+    s.Wn[j * nc + c] = s.W[j * nc + c];
+    s.Un[j * nc + c] = s.U[j * nc + c];
+  }
+  if (F::sst) {
+    for (int q = 0; q < 2; q++) {
+      s.tdW[q * nc + c] = 0.0;
+      s.tWn[q * nc + c] = s.tW[q * nc + c];
+      s.tUn[q * nc + c] = s.tU[q * nc + c];
+    }
+    s.mutn[c] = s.mut[c];
+  }
 }
 
 template <int FLOW, typename T, bool B>
@@ -595,21 +636,13 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
     // a batch case whose residual failed a check in this iteration keeps
     // its state (the other cases are independent of it)
     if (s.err[2 + m.ccase[c]]) {
-      for (int j = 0; j < 4; j++) {
-        s.dW[j * nc + c] = 0.0;
-        s.Wn[j * nc + c] = s.W[j * nc + c];
-        s.Un[j * nc + c] = s.U[j * nc + c];
-      }
-      if (F::sst) {
-        for (int q = 0; q < 2; q++) {
-          s.tdW[q * nc + c] = 0.0;
-          s.tWn[q * nc + c] = s.tW[q * nc + c];
-          s.tUn[q * nc + c] = s.tU[q * nc + c];
-        }
-        s.mutn[c] = s.mut[c];
-      }
+      carry_state<FLOW>(m, s, c);
       return;
     }
+  }
+  if (frozen_cell(m, c)) {
+    carry_state<FLOW>(m, s, c);
+    return;
   }
   double dW[4], Wn[4], Un[4];
   for (int j = 0; j < 4; j++) {
@@ -659,14 +692,11 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
 // ViscousFlow2D.h:223-242), D (SolveDiag, TopologyHolderStrategy.cpp:190-215),
 // the SST diagonal (SSTKWStrategy.h:170-192) and mu/Pr of the cell for r_A.
 // ---------------------------------------------------------------------------
-template <int FLOW, typename T, bool B>
-__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k_,
-                                                const int *__restrict__ items, const XSlice sl) {
+template <int FLOW, typename T, bool B, typename In>
+__device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, const Consts &k_, int c, In in) {
   using F = Flow<FLOW>;
-  int c;
-  if (!xcd_item(items, sl, c)) return;
   const Consts k = case_consts<B>(m, k_, c);
-  const int nc = m.nc, nf = m.nf;
+  const int nc = m.nc;
   int fs[4];
   int ns = 0;
   for (int sl = 0; sl < 4; sl++) {
@@ -676,40 +706,37 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   }
   // faces in ascending order, each added with this cell's sign (left +,
   // right -): the reference's scatter order.  Sums are FP64 in every mode.
-  const T *const FFb = pickc<T>(s.FF, s.FFf);
   double Rc[4] = {0, 0, 0, 0};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
-    const T *FF = FFb + (e >= 0 ? e : -e - 1);
     if (e >= 0) {
-      for (int j = 0; j < 4; j++) Rc[j] += double(FF[(size_t)j * nf]);
+      for (int j = 0; j < 4; j++) Rc[j] += in(i, e, j);
     } else {
-      for (int j = 0; j < 4; j++) Rc[j] -= double(FF[(size_t)j * nf]);
+      for (int j = 0; j < 4; j++) Rc[j] -= in(i, e, j);
     }
   }
   if (F::visc) {
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
-      const T *FF = FFb + (e >= 0 ? e : -e - 1);
       if (e >= 0) {
-        for (int j = 0; j < 4; j++) Rc[j] -= double(FF[(size_t)(5 + j) * nf]);
+        for (int j = 0; j < 4; j++) Rc[j] -= in(i, e, 5 + j);
       } else {
-        for (int j = 0; j < 4; j++) Rc[j] += double(FF[(size_t)(5 + j) * nf]);
+        for (int j = 0; j < 4; j++) Rc[j] += in(i, e, 5 + j);
       }
     }
   }
   const double V = m.vol[c];
-  double lamc = 0, lamv = 0;
-  for (int i = 0; i < ns; i++) {
-    const int e = fs[i];
-    lamc += double(FFb[(e >= 0 ? e : -e - 1) + 4 * (size_t)nf]);
+  if (k.dts[0] != 0.0) {
+    // dual time stepping: the physical-time derivative of W by backward
+    // differences, added after the face fluxes
+    for (int j = 0; j < 4; j++)
+      Rc[j] += V * (k.dts[0] * s.W[j * nc + c] + k.dts[1] * s.Wt1[j * nc + c] + k.dts[2] * s.Wt2[j * nc + c]);
   }
+  double lamc = 0, lamv = 0;
+  for (int i = 0; i < ns; i++) lamc += in(i, fs[i], 4);
   s.lamc[c] = lamc;
   if (F::visc) {
-    for (int i = 0; i < ns; i++) {
-      const int e = fs[i];
-      lamv += double(FFb[(e >= 0 ? e : -e - 1) + 9 * (size_t)nf]) / V;
-    }
+    for (int i = 0; i < ns; i++) lamv += in(i, fs[i], 9) / V;
     s.lamv[c] = lamv;
   }
   store4(s.R, nc, c, Rc);
@@ -722,6 +749,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
   s.dt[c] = dtc;
   double diag = V / dtc + 0.5 * OMEGAN * lamc;
   if (F::visc) diag += lamv;
+  if (k.dts[0] != 0.0) diag += k.dts[0] * V;
   s.D[c] = diag;
   if (F::sst) {
     double tUc[2] = {s.tU[c], s.tU[nc + c]};
@@ -745,13 +773,12 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
     double trc[2] = {-(Q0 * V), -(Q1 * V)};
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
-      const T *FF = FFb + (e >= 0 ? e : -e - 1);
       if (e >= 0) {
-        trc[0] += double(FF[10 * (size_t)nf]);
-        trc[1] += double(FF[11 * (size_t)nf]);
+        trc[0] += in(i, e, 10);
+        trc[1] += in(i, e, 11);
       } else {
-        trc[0] -= double(FF[10 * (size_t)nf]);
-        trc[1] -= double(FF[11 * (size_t)nf]);
+        trc[0] -= in(i, e, 10);
+        trc[1] -= in(i, e, 11);
       }
     }
     s.tR[c] = trc[0];
@@ -770,6 +797,65 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
     fwd_store<FLOW>(m, s, c, zero, zero, diag, dW1, tdW1);
     if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   }
+}
+
+
+// The face record in global memory (written by k_face), read by slot.
+template <typename T>
+struct FaceInGlobal {
+  const T *FF;
+  int nf;
+  __device__ __forceinline__ double operator()(int, int e, int q) const {
+    return double(FF[(e >= 0 ? e : -e - 1) + (size_t)q * nf]);
+  }
+};
+
+template <int FLOW, typename T, bool B>
+__global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState s, const Consts k_,
+                                                const int *__restrict__ items, const XSlice sl) {
+  int c;
+  if (!xcd_item(items, sl, c)) return;
+  cell_terms<FLOW, T, B>(m, s, k_, c, FaceInGlobal<T>{pickc<T>(s.FF, s.FFf), m.nf});
+}
+
+// ---------------------------------------------------------------------------
+// k_facecell: face_terms and cell_terms in one launch, the face record held in
+// LDS instead of a round trip through HBM (FP64).  Launch block b serves the
+// cells k_cell's block b serves (a chunk of the part-grouped Hilbert list:
+// spatially compact, of every colour); its tile's faces -- every face of
+// those cells, ascending -- are computed first, each into LDS slot
+// lds[q*stride + i], then each cell gathers its faces from LDS in ascending
+// face order, as k_cell gathers them from the face record.  A face between
+// two tiles is computed by both (the same bits), so no tile waits for
+// another.  Results equal k_face + k_cell bit for bit.
+// ---------------------------------------------------------------------------
+struct FaceOutLds {
+  double *L;
+  int stride, i;
+  __device__ __forceinline__ void operator()(int q, double v) const { L[q * stride + i] = v; }
+};
+struct FaceInLds {
+  const double *L;
+  int stride;
+  const int *lf;  // local face index of each slot of the cell
+  __device__ __forceinline__ double operator()(int i, int, int q) const { return L[q * stride + lf[i]]; }
+};
+
+template <int FLOW, bool B>
+__global__ __launch_bounds__(kBlock) void k_facecell(const DevMesh m, const DevState s, const Consts k_,
+                                                    const int *__restrict__ items, const XSlice sl,
+                                                    const int *__restrict__ tfoff, const int *__restrict__ tface,
+                                                    const unsigned short *__restrict__ lfs, int stride) {
+  extern __shared__ double lds[];
+  const int t = blockIdx.x, f0 = tfoff[t], f1 = tfoff[t + 1];
+  for (int i = f0 + (int)threadIdx.x; i < f1; i += (int)blockDim.x)
+    face_terms<FLOW, double, B>(m, s, k_, tface[i], FaceOutLds{lds, stride, i - f0});
+  __syncthreads();
+  int c;
+  if (!xcd_item(items, sl, c)) return;
+  int lf[4];
+  for (int q = 0; q < 4; q++) lf[q] = lfs[(size_t)q * m.nc + c];
+  cell_terms<FLOW, double, B>(m, s, k_, c, FaceInLds{lds, stride, lf});
 }
 
 // ---------------------------------------------------------------------------
@@ -946,7 +1032,7 @@ __device__ void norm_partial_block(const DevMesh &m, const double *R, const int 
   __shared__ double sh[kBlock];
   const int c = b0[b] + threadIdx.x;
   double v = 0;
-  if (c < b1[b]) {
+  if (c < b1[b] && !frozen_cell(m, c)) {
     double r = R[c] / m.vol[c];
     v = r * r;
   }
@@ -1003,25 +1089,73 @@ __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-// Wall force: sum over wall faces of (p n A) from the left cell's boundary state.
+// Viscous momentum flux tau . n A of wall face f of its cell c, from the
+// current state: the cell's Green-Gauss velocity gradient with k_grad's
+// arithmetic and the wall face's stress with face_terms' (FP64), so the
+// result is what a residual evaluation of this state would put in the face
+// record -- without touching the iteration's arrays.
+__device__ void wall_shear(const DevMesh &m, const DevState &s, const Consts &k, int f, int c, double &gx,
+                           double &gy) {
+  const int nc = m.nc;
+  double Ucd[4];
+  load4(s.U, nc, c, Ucd);
+  double g[4] = {0, 0, 0, 0};  // du/dx, du/dy, dv/dx, dv/dy
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
+    double Ue[4];
+    if (o.nb >= 0) {
+      double Unbd[4];
+      load4(s.U, nc, o.nb, Unbd);
+      for (int j = 2; j < 4; j++) Ue[j] = o.left ? (Ucd[j] + Unbd[j]) / 2 : (Unbd[j] + Ucd[j]) / 2;
+    } else {
+      boundary_state<true>(k, o.nb, Ucd, m.fnx[o.f], m.fny[o.f], Ue);
+    }
+    for (int j = 0; j < 2; j++) {
+      if (o.left) {
+        g[2 * j] = g[2 * j] + n0 * (A * Ue[2 + j]);
+        g[2 * j + 1] = g[2 * j + 1] + n1 * (A * Ue[2 + j]);
+      } else {
+        g[2 * j] = g[2 * j] - n0 * (A * Ue[2 + j]);
+        g[2 * j + 1] = g[2 * j + 1] - n1 * (A * Ue[2 + j]);
+      }
+    }
+  }
+  const double V = m.vol[c];
+  for (int j = 0; j < 4; j++) g[j] = g[j] / V;
+  double Ue[4];
+  boundary_state<true>(k, kWallD, Ucd, m.fnx[f], m.fny[f], Ue);
+  const double mu = sutherland<double>(k, Ue[1] / Ue[0]);
+  const double n0 = m.fnx[f], n1 = m.fny[f], A = m.fA[f];
+  const double efficient = 2.0 / 3.0 * mu;
+  const double taoxx = efficient * (2.0 * g[0] - g[3]);
+  const double taoyy = efficient * (2.0 * g[3] - g[0]);
+  const double taoxy = mu * (g[1] + g[2]);
+  gx = (taoxx * n0 + taoxy * n1) * A;
+  gy = (taoxy * n0 + taoyy * n1) * A;
+}
+
 // Wall force over each case's wall faces (one block per case: wall faces
 // [woff[case], woff[case+1])): p n A with the wall cell's pressure, minus
-// the face record's viscous momentum flux for viscous flow.
-template <typename T, bool VISC>
-__global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const int *wall_faces,
-                                                   const int *wall_cells, const int *woff, double *out2) {
+// the viscous momentum flux for viscous flow, all of the current state.
+template <bool VISC>
+__global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevState s, const Consts k_,
+                                                   const int *wall_faces, const int *wall_cells, const int *woff,
+                                                   double *out2) {
   __shared__ double sx[kBlock], sy[kBlock];
   double fx = 0, fy = 0;
   const int i0 = woff[blockIdx.x], nw = woff[blockIdx.x + 1] - i0;
-  const int nf = m.nf;
-  const T *FF = pickc<T>(s.FF, s.FFf);
   for (int i = threadIdx.x; i < nw; i += kBlock) {
     int f = wall_faces[i0 + i], c = wall_cells[i0 + i];
     double p = s.U[m.nc + c];
     double gx = p * m.fnx[f] * m.fA[f], gy = p * m.fny[f] * m.fA[f];
     if (VISC) {
-      gx = gx - double(FF[7 * (size_t)nf + f]);
-      gy = gy - double(FF[8 * (size_t)nf + f]);
+      const Consts k = m.kc ? m.kc[m.ccase[c]] : k_;
+      double vx, vy;
+      wall_shear(m, s, k, f, c, vx, vy);
+      gx = gx - vx;
+      gy = gy - vy;
     }
     fx = fx + gx;
     fy = fy + gy;
@@ -1063,6 +1197,82 @@ __global__ __launch_bounds__(kBlock) void k_stencil(const double *src, double *d
     acc += src[column ? (long)kv * sp + d : (long)d * nvar + kv] * w[4 * r + mm];
   }
   dst[column ? (long)kv * dp + rc : (long)rc * nvar + kv] = acc;
+}
+
+// Overset receptor cells from their donors, at the start of an iteration:
+// U_r = sum_k w_k U_{d_k} over the receptor's donor cells (ascending), W_r
+// = cons(U_r).  The stencil composes the reference's coupling stages --
+// field-cell values averaged to the donor mesh's nodes (as Euler2D::outPutCp
+// averages, Euler2D.cpp:396-422), the nodal donor stencil
+// (MeshBlock::getInterpolatedSolution, UniTioga/dataUpdate.cpp:6-113) and
+// the receptor cell as the mean of its vertices (Orchestra.cpp:269-292) --
+// into one linear map.  Donors are field cells, which this kernel never
+// writes.
+__global__ __launch_bounds__(kBlock) void k_overset(const DevMesh m, const DevState s, int n, const int *recv,
+                                                    const int *off, const int *donor, const double *w) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int nc = m.nc, c = recv[r];
+  double U[4] = {0.0, 0.0, 0.0, 0.0}, W[4];
+  for (int i = off[r]; i < off[r + 1]; i++) {
+    const int d = donor[i];
+    const double wi = w[i];
+    for (int j = 0; j < 4; j++) U[j] += wi * s.U[j * nc + d];
+  }
+  cons_of(U, W);
+  store4(s.U, nc, c, U);
+  store4(s.W, nc, c, W);
+}
+
+// Face states of the current U (Euler2D::updateEdgeValues through
+// BoundaryStrategy::HandleBoundary, BoundaryStrategy.h:73-335): the mean of
+// the two cells inside, the wall or far-field state at boundary faces; out
+// is [4][nf].  Computed on request for the U_edge getter.
+template <bool VISC>
+__global__ __launch_bounds__(kBlock) void k_uedge(const DevMesh m, const DevState s, const Consts k_, double *out) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nc = m.nc, nf = m.nf;
+  if (f >= nf) return;
+  const int L = m.flc[f], R = m.frc[f];
+  const Consts k = m.kc ? m.kc[m.ccase[L]] : k_;
+  double UL[4], Ue[4];
+  load4(s.U, nc, L, UL);
+  if (R >= 0) {
+    double UR[4];
+    load4(s.U, nc, R, UR);
+    for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
+  } else {
+    boundary_state<VISC>(k, R, UL, m.fnx[f], m.fny[f], Ue);
+  }
+  for (int j = 0; j < 4; j++) out[(size_t)j * nf + f] = Ue[j];
+}
+
+// The SST model's wall distance of every cell: the Euclidean distance from
+// the cell centre to the nearest wall face segment of its case, seg[4*i ..]
+// = (x0, y0, x1, y1) of wall face i, faces [woff[case], woff[case+1]).  The
+// reference's projected distance (UnstructBlock.h:265-298) is small near
+// the tangent line of any wall face, far from the wall too; its SST, never
+// executed there, would blend most of the far field to the near-wall
+// branch.
+__global__ __launch_bounds__(kBlock) void k_wall_distance(int nc, const double *ccx, const double *ccy,
+                                                          const int *ccase, const int *woff, const double *seg,
+                                                          double *out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int cs = ccase ? ccase[c] : 0;
+  const double cx = ccx[c], cy = ccy[c];
+  double d = 100000000000.0;
+  for (int i = woff[cs]; i < woff[cs + 1]; i++) {
+    const double x0 = seg[4 * i], y0 = seg[4 * i + 1], x1 = seg[4 * i + 2], y1 = seg[4 * i + 3];
+    double ax = x1 - x0, ay = y1 - y0;
+    double px = cx - x0, py = cy - y0;
+    double t = (px * ax + py * ay) / (ax * ax + ay * ay);
+    if (!(t > 0.0)) t = 0.0;
+    if (t > 1.0) t = 1.0;
+    double qx = px - t * ax, qy = py - t * ay;
+    d = smin(d, sqrt(qx * qx + qy * qy));
+  }
+  out[c] = d;
 }
 
 // Plain copy of n doubles: the reference kernel for calibrating traffic counters.
@@ -1138,6 +1348,23 @@ struct usg_ctx {
   int *d_nb0 = nullptr, *d_nb1 = nullptr, *d_nboff = nullptr, *d_woff = nullptr;
   // threads per block of the kernel groups (USG_K_GRAD .. USG_K_SWEEP)
   int blk[4] = {kBlock, kBlock, kBlock, kBlock};
+  // overset coupling: receptor cells and their donor stencil (CSR), applied
+  // at the start of every LU-SGS iteration; the blanking is dm.blank
+  int ov_n = 0;
+  int *d_ov_recv = nullptr, *d_ov_off = nullptr, *d_ov_donor = nullptr;
+  double *d_ov_w = nullptr;
+  signed char *d_blank = nullptr;
+  // dual time stepping: physical step, order, physical steps taken
+  double dt_phys = 0.0;
+  int dt_order = 0, dt_steps = 0;
+  // fused face-cell launch (k_facecell, FP64): tiles of tile_bs cells, their
+  // faces (offsets, ids), the local face index of every cell slot, and the
+  // LDS row length (the most faces of a tile)
+  double *d_uedge = nullptr;  // face states for the U_edge getter, [4][nf]
+  bool fused = true;
+  int tile_bs = 0, tile_stride = 0;
+  int *d_tfoff = nullptr, *d_tface = nullptr;
+  unsigned short *d_lfs = nullptr;
 };
 
 constexpr int kRing = 4096;
@@ -1179,6 +1406,7 @@ static void freestream(const usg_params &prm, Consts &kk, double &aoa_rad) {
   kk.pr = 0.72;
   kk.prt = 0.9;
   kk.cfl = prm.cfl_lusgs;
+  kk.order = 2;
   aoa_rad = fs_AOA;
   // SSTkomegaModel::InitializeturbulenceVariable (SSTKWStrategy.h:582-613)
   const double L = 1.0;
@@ -1208,12 +1436,93 @@ static int xgrid(const XSlice &sl, int bs) {
   return kXcd * std::max(1, (mx + bs - 1) / bs);
 }
 
+// quantities of a face record: flux 4 + spectral 1 (+ viscous flux 4 +
+// viscous spectral 1) (+ SST 2)
+template <int FLOW>
+constexpr int face_q() { return FLOW == USG_SST ? 12 : (FLOW == USG_LAMINAR ? 10 : 5); }
+
+constexpr size_t kMaxTileLds = 64 * 1024;
+
+// The fused launch's tiles for blocks of bs cells: tile t is the cells k_cell's
+// block t serves; its faces are every face of those cells, ascending.  False
+// when a tile's face record would not fit the LDS a block may take.
+static bool build_tiles(usg_ctx *c, int bs) {
+  if (c->tile_bs == bs && c->d_tfoff) return true;
+  const HostMesh &h = c->hm;
+  const int nc = h.nc;
+  const XSlice cs = slice(h.cell_off.data());
+  const int ntile = xgrid(cs, bs);
+  std::vector<int> tfoff(ntile + 1, 0), tface, faces;
+  std::vector<unsigned short> lfs(4 * (size_t)nc, 0);
+  int maxf = 0;
+  for (int t = 0; t < ntile; t++) {
+    const int p = t % kXcd, ch = t / kXcd;
+    const int a = cs.off[p] + ch * bs, b = std::min(a + bs, cs.off[p + 1]);
+    faces.clear();
+    for (int i = a; i < b; i++) {
+      const int cell = h.cell_list[i];
+      for (int q = 0; q < h.cnf[cell]; q++) faces.push_back(h.cface[4 * cell + q]);
+    }
+    std::sort(faces.begin(), faces.end());
+    faces.erase(std::unique(faces.begin(), faces.end()), faces.end());
+    if (faces.size() > 65535) return false;
+    for (int i = a; i < b; i++) {
+      const int cell = h.cell_list[i];
+      for (int q = 0; q < h.cnf[cell]; q++)
+        lfs[(size_t)q * nc + cell] =
+            (unsigned short)(std::lower_bound(faces.begin(), faces.end(), h.cface[4 * cell + q]) - faces.begin());
+    }
+    tface.insert(tface.end(), faces.begin(), faces.end());
+    tfoff[t + 1] = (int)tface.size();
+    maxf = std::max(maxf, (int)faces.size());
+  }
+  const int nq = c->prm.flow == USG_SST ? 12 : (c->prm.flow == USG_LAMINAR ? 10 : 5);
+  if ((size_t)nq * std::max(maxf, 1) * sizeof(double) > kMaxTileLds) return false;
+  for (void *q : {(void *)c->d_tfoff, (void *)c->d_tface, (void *)c->d_lfs}) {
+    if (!q) continue;
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    (void)hipFree(q);
+  }
+  c->d_tfoff = c->d_tface = nullptr;
+  c->d_lfs = nullptr;
+  if (upload(c, &c->d_tfoff, tfoff.data(), tfoff.size()) ||
+      upload(c, &c->d_tface, tface.data(), std::max<size_t>(tface.size(), 1)) ||
+      upload(c, &c->d_lfs, lfs.data(), lfs.size()))
+    return false;
+  c->tile_bs = bs;
+  c->tile_stride = std::max(maxf, 1);
+  return hipStreamSynchronize(c->stream) == hipSuccess;
+}
+
+// The fused face-cell launch is taken for FP64 when its tiles fit.  Tiles
+// are (re)built eagerly, never while the stream is being captured.
+static bool fused_ready(usg_ctx *c) {
+  if (!c->fused || c->prm.precision != USG_FP64) return false;
+  if (c->tile_bs == c->blk[USG_K_CELL] && c->d_tfoff) return true;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(c->stream, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+  if (build_tiles(c, c->blk[USG_K_CELL])) return true;
+  c->fused = false;
+  return false;
+}
+
 template <int FLOW, typename T, bool B>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
   const int bg = c->blk[USG_K_GRAD], bf = c->blk[USG_K_FACE], bc = c->blk[USG_K_CELL];
   hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs, bg)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
+  if constexpr (std::is_same<T, double>::value) {
+    if (fused_ready(c)) {
+      if (ea) hipEventRecord(ea, c->stream);
+      const size_t lds = (size_t)face_q<FLOW>() * c->tile_stride * sizeof(double);
+      hipLaunchKernelGGL((k_facecell<FLOW, B>), dim3(xgrid(cs, bc)), dim3(bc), lds, c->stream, c->dm, c->ds, c->k,
+                         c->d_cell_list, cs, c->d_tfoff, c->d_tface, c->d_lfs, c->tile_stride);
+      if (eb) hipEventRecord(eb, c->stream);
+      return;
+    }
+  }
   if (ea) hipEventRecord(ea, c->stream);
   hipLaunchKernelGGL((k_face<FLOW, T, B>), dim3(xgrid(fs, bf)), dim3(bf), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_face_list, fs);
@@ -1505,7 +1814,7 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, (double **)&dm.vol, h.vol.data(), nc));
   CK(upload(c, (double **)&dm.ccx, ccx.data(), nc));
   CK(upload(c, (double **)&dm.ccy, ccy.data(), nc));
-  CK(upload(c, (double **)&dm.wdist, h.wdist.data(), nc));
+  if (params->flow != USG_SST) CK(upload(c, (double **)&dm.wdist, h.wdist.data(), nc));
   CK(upload(c, (int **)&dm.cf, cf.data(), cf.size()));
   CK(upload(c, (int **)&dm.cnb, cnb.data(), cnb.size()));
   CK(upload(c, (int **)&dm.fslot, fslot.data(), fslot.size()));
@@ -1647,6 +1956,24 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
       CK(upload(c, (Consts **)&dm.kc, c->ck.data(), c->ck.size()));
     }
   }
+  if (params->flow == USG_SST) {
+    // the SST model's wall distance, computed here from the wall segments
+    std::vector<double> seg(4 * std::max<size_t>(h.wall_faces.size(), 1), 0.0);
+    for (size_t i = 0; i < h.wall_faces.size(); i++) {
+      const int f = h.wall_faces[i];
+      for (int q = 0; q < 2; q++) {
+        seg[4 * i + 2 * q] = h.xy[2 * (size_t)h.fnode[2 * f + q]];
+        seg[4 * i + 2 * q + 1] = h.xy[2 * (size_t)h.fnode[2 * f + q] + 1];
+      }
+    }
+    double *dseg = nullptr, *dwd = nullptr;
+    CK(upload(c, &dseg, seg.data(), seg.size()));
+    CK(dalloc(c, &dwd, nc));
+    hipLaunchKernelGGL(k_wall_distance, dim3(grid(nc)), dim3(kBlock), 0, c->stream, nc, dm.ccx, dm.ccy, dm.ccase,
+                       c->d_woff, dseg, dwd);
+    if (hipGetLastError() != hipSuccess) CK(fail(USG_ERR_HIP, "wall distance launch"));
+    dm.wdist = dwd;
+  }
   CK(dalloc(c, &c->d_partial, c->nnb));
   CK(dalloc(c, &c->d_force, 2 * (size_t)c->ncase));
   CK(ensure_norms(c, 16 * c->ncase));
@@ -1781,7 +2108,14 @@ static bool fuse_norm(const usg_ctx *c) {
 
 // One LU-SGS iteration (LUSGSIntegrator::UpdateSerial): residual, norm into
 // the ring, sweeps (the backward sweep writes the update), buffer swap.
+static void launch_overset(usg_ctx *c) {
+  if (c->ov_n > 0)
+    hipLaunchKernelGGL(k_overset, dim3(grid(c->ov_n)), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ov_n,
+                       c->d_ov_recv, c->d_ov_off, c->d_ov_donor, c->d_ov_w);
+}
+
 static void one_iteration(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
+  launch_overset(c);
   launch_residual_events(c, ea, eb);
   const bool fuse = fuse_norm(c);
   if (!fuse) launch_norm_ring(c);
@@ -1831,6 +2165,7 @@ static bool build_graph(usg_ctx *c, int slot) {
 static bool ensure_graphs(usg_ctx *c) {
   if (c->gexec[0] && c->gexec[1] && c->gstream[0] == c->stream && c->gstream[1] == c->stream) return true;
   drop_graphs(c);
+  (void)fused_ready(c);  // the fused launch's tiles, before capture
   if (!build_graph(c, 0)) return false;
   swap_state(c);
   const bool ok = build_graph(c, 1);
@@ -1876,6 +2211,8 @@ int usg_lusgs_step(usg_ctx *c, int n, double *res_hist) {
 int usg_rk5_step(usg_ctx *c, int n, double *res_hist) {
   if (!c || n < 0) return fail(USG_ERR_ARG, "bad argument");
   if (c->prm.flow == USG_SST) return fail(USG_ERR_ARG, "RK5 is provided for Euler and laminar flow");
+  if (c->dm.blank || c->dt_phys > 0.0)
+    return fail(USG_ERR_ARG, "RK5 is provided without overset coupling and dual time stepping");
   if (res_hist) {
     int rc = ensure_norms(c, n * c->ncase);
     if (rc) return rc;
@@ -1910,8 +2247,8 @@ static int soa_to_aos(usg_ctx *c, const float *dev, int nv, double *out, long co
   return USG_OK;
 }
 
-static int soa_to_aos(usg_ctx *c, const double *dev, int nv, double *out, long count, bool has) {
-  const int nc = c->hm.nc;
+static int soa_to_aos(usg_ctx *c, const double *dev, int nv, double *out, long count, bool has, int nitems = -1) {
+  const int nc = nitems >= 0 ? nitems : c->hm.nc;
   if (count != (long)nv * nc) return fail(USG_ERR_ARG, "count mismatch");
   if (!has || !dev) return fail(USG_ERR_ARG, "field not available for this flow type");
   std::vector<double> tmp((size_t)nv * nc);
@@ -1948,6 +2285,19 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
     case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
     case USG_F_TCONS: return soa_to_aos(c, s.tW, 2, out, count, sst);
     case USG_F_W0: return soa_to_aos(c, s.W0, 4, out, count, true);
+    case USG_F_WDIST: return soa_to_aos(c, const_cast<double *>(c->dm.wdist), 1, out, count, sst);
+    case USG_F_UEDGE: {
+      const int nf = c->hm.nf;
+      if (count != 4L * nf) return fail(USG_ERR_ARG, "count mismatch");
+      if (!c->d_uedge) {
+        int rc = dalloc(c, &c->d_uedge, 4 * (size_t)nf);
+        if (rc) return rc;
+      }
+      hipLaunchKernelGGL((visc ? k_uedge<true> : k_uedge<false>), dim3((nf + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                         c->stream, c->dm, c->ds, c->k, c->d_uedge);
+      HIPCHK(hipGetLastError());
+      return soa_to_aos(c, c->d_uedge, 4, out, count, true, nf);
+    }
     default: return fail(USG_ERR_ARG, "unknown field");
   }
 }
@@ -1964,20 +2314,10 @@ int usg_res_l2(usg_ctx *c, double *out) {
 
 int usg_forces(usg_ctx *c, double *cl, double *cd) {
   if (!c || !cl || !cd) return fail(USG_ERR_ARG, "null argument");
-  const bool visc = c->prm.flow != USG_EULER, mixed = c->prm.precision == USG_MIXED;
-  if (visc) {
-    // the viscous wall flux of the current state: one residual evaluation
-    launch_residual_events(c, nullptr, nullptr);
-    c->have_residual = true;
-  }
-  if (mixed)
-    hipLaunchKernelGGL((visc ? k_forces<float, true> : k_forces<float, false>), dim3(c->ncase), dim3(kBlock), 0,
-                       c->stream, c->dm, c->ds, c->d_wall_faces, c->d_wall_cells, c->d_woff, c->d_force);
-  else
-    hipLaunchKernelGGL((visc ? k_forces<double, true> : k_forces<double, false>), dim3(c->ncase), dim3(kBlock), 0,
-                       c->stream, c->dm, c->ds, c->d_wall_faces, c->d_wall_cells, c->d_woff, c->d_force);
-  int rc = check_err(c);
-  if (rc) return rc;
+  const bool visc = c->prm.flow != USG_EULER;
+  hipLaunchKernelGGL((visc ? k_forces<true> : k_forces<false>), dim3(c->ncase), dim3(kBlock), 0, c->stream, c->dm,
+                     c->ds, c->k, c->d_wall_faces, c->d_wall_cells, c->d_woff, c->d_force);
+  HIPCHK(hipGetLastError());
   std::vector<double> f(2 * (size_t)c->ncase);
   HIPCHK(hipMemcpyAsync(f.data(), c->d_force, f.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2043,6 +2383,151 @@ int usg_mesh_build(const usg_mesh_desc *mesh, double *vol, double *cc, double *f
   if (wdist) std::memcpy(wdist, h.wdist.data(), h.wdist.size() * sizeof(double));
   if (nfwd) *nfwd = (int)h.fwd_off.size() - 1;
   if (nbwd) *nbwd = (int)h.bwd_off.size() - 1;
+  return USG_OK;
+}
+
+static void dfree(usg_ctx *c, void *p) {
+  if (!p) return;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  (void)hipFree(p);
+}
+
+int usg_set_overset(usg_ctx *c, const int *blank, int nrecv, const int *recv, const int *off, const int *donor,
+                    const double *w) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  dfree(c, c->d_ov_recv);
+  dfree(c, c->d_ov_off);
+  dfree(c, c->d_ov_donor);
+  dfree(c, c->d_ov_w);
+  dfree(c, c->d_blank);
+  c->d_ov_recv = c->d_ov_off = c->d_ov_donor = nullptr;
+  c->d_ov_w = nullptr;
+  c->d_blank = nullptr;
+  c->dm.blank = nullptr;
+  c->ov_n = 0;
+  if (!blank) return USG_OK;
+  if (c->prm.flow == USG_SST) return fail(USG_ERR_ARG, "overset coupling is provided for Euler and laminar flow");
+  const int nc = c->hm.nc;
+  if (nrecv < 0 || nrecv > nc || (nrecv && (!recv || !off || !donor || !w)))
+    return fail(USG_ERR_ARG, "bad overset stencil arguments");
+  std::vector<signed char> b(nc);
+  for (int i = 0; i < nc; i++) {
+    if (blank[i] < -1 || blank[i] > 1) return fail(USG_ERR_ARG, "cell blanking must be 1, 0 or -1");
+    b[i] = (signed char)blank[i];
+  }
+  // receptors: distinct cells blanked -1; donors: field cells, which the
+  // interpolation never writes
+  std::vector<char> seen(nc, 0);
+  if (nrecv && off[0] != 0) return fail(USG_ERR_ARG, "donor offsets must start at 0");
+  for (int r = 0; r < nrecv; r++) {
+    if (recv[r] < 0 || recv[r] >= nc || b[recv[r]] != -1 || seen[recv[r]])
+      return fail(USG_ERR_ARG, "receptor " + std::to_string(r) + " is not a distinct receptor cell");
+    seen[recv[r]] = 1;
+    if (off[r + 1] < off[r]) return fail(USG_ERR_ARG, "donor offsets must not decrease");
+    for (int i = off[r]; i < off[r + 1]; i++)
+      if (donor[i] < 0 || donor[i] >= nc || b[donor[i]] != 1 || !std::isfinite(w[i]))
+        return fail(USG_ERR_ARG, "receptor " + std::to_string(r) + " has a donor that is not a field cell");
+  }
+  int rc;
+  if ((rc = upload(c, &c->d_blank, b.data(), nc))) return rc;
+  c->dm.blank = c->d_blank;
+  if (nrecv) {
+    const int nd = off[nrecv];
+    if ((rc = upload(c, &c->d_ov_recv, recv, nrecv)) || (rc = upload(c, &c->d_ov_off, off, nrecv + 1)) ||
+        (rc = upload(c, &c->d_ov_donor, donor, nd)) || (rc = upload(c, &c->d_ov_w, w, nd)))
+      return rc;
+  }
+  c->ov_n = nrecv;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return USG_OK;
+}
+
+// Backward-difference coefficients of the current physical step into every
+// case's constants (graphs captured the old ones: dropped when they change).
+static int set_dual_coefficients(usg_ctx *c, double c0, double c1, double c2) {
+  const double v[3] = {c0, c1, c2};
+  bool changed = false;
+  for (int q = 0; q < 3; q++) changed |= c->k.dts[q] != v[q];
+  if (!changed) return USG_OK;
+  for (int q = 0; q < 3; q++) {
+    c->k.dts[q] = v[q];
+    for (auto &kk : c->ck) kk.dts[q] = v[q];
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  if (c->dm.kc)
+    HIPCHK(hipMemcpy((void *)c->dm.kc, c->ck.data(), c->ck.size() * sizeof(Consts), hipMemcpyHostToDevice));
+  return USG_OK;
+}
+
+int usg_set_cfl(usg_ctx *c, double cfl) {
+  if (!c || !(cfl > 0.0) || !std::isfinite(cfl)) return fail(USG_ERR_ARG, "bad CFL number");
+  if (c->k.cfl == cfl) return USG_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->k.cfl = cfl;
+  for (auto &kk : c->ck) kk.cfl = cfl;
+  for (auto &p : c->cprm) p.cfl_lusgs = cfl;
+  c->prm.cfl_lusgs = cfl;
+  if (c->dm.kc)
+    HIPCHK(hipMemcpy((void *)c->dm.kc, c->ck.data(), c->ck.size() * sizeof(Consts), hipMemcpyHostToDevice));
+  return USG_OK;
+}
+
+int usg_set_order(usg_ctx *c, int order) {
+  if (!c || (order != 1 && order != 2)) return fail(USG_ERR_ARG, "order must be 1 or 2");
+  if (c->k.order == order) return USG_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->k.order = order;
+  for (auto &kk : c->ck) kk.order = order;
+  if (c->dm.kc)
+    HIPCHK(hipMemcpy((void *)c->dm.kc, c->ck.data(), c->ck.size() * sizeof(Consts), hipMemcpyHostToDevice));
+  return USG_OK;
+}
+
+int usg_dual_time(usg_ctx *c, double dt_phys, int order) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  if (dt_phys > 0.0 && c->prm.flow == USG_SST)
+    return fail(USG_ERR_ARG, "dual time stepping is provided for Euler and laminar flow");
+  if (dt_phys > 0.0 && order != 1 && order != 2) return fail(USG_ERR_ARG, "order must be 1 or 2");
+  if (dt_phys > 0.0 && !c->ds.Wt1) {
+    int rc;
+    if ((rc = dalloc(c, &c->ds.Wt1, 4 * (size_t)c->hm.nc)) || (rc = dalloc(c, &c->ds.Wt2, 4 * (size_t)c->hm.nc)))
+      return rc;
+    drop_graphs(c);   // the state view the graphs captured changed
+  }
+  c->dt_phys = dt_phys > 0.0 ? dt_phys : 0.0;
+  c->dt_order = order;
+  c->dt_steps = 0;
+  return set_dual_coefficients(c, 0.0, 0.0, 0.0);
+}
+
+int usg_dual_time_advance(usg_ctx *c) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  if (!(c->dt_phys > 0.0)) return fail(USG_ERR_ARG, "dual time stepping is off (usg_dual_time)");
+  const size_t bytes = 4 * (size_t)c->hm.nc * sizeof(double);
+  c->dt_steps++;
+  const bool bdf2 = c->dt_order == 2 && c->dt_steps > 1;
+  if (bdf2) HIPCHK(hipMemcpyAsync(c->ds.Wt2, c->ds.Wt1, bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(c->ds.Wt1, c->ds.W, bytes, hipMemcpyDeviceToDevice, c->stream));
+  const double dt = c->dt_phys;
+  return bdf2 ? set_dual_coefficients(c, 1.5 / dt, -2.0 / dt, 0.5 / dt)
+              : set_dual_coefficients(c, 1.0 / dt, -1.0 / dt, 0.0);
+}
+
+int usg_use_fused(usg_ctx *c, int on) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->fused = on != 0;
+  if (c->fused && c->prm.precision == USG_FP64 && !build_tiles(c, c->blk[USG_K_CELL])) {
+    c->fused = false;
+    return fail(USG_ERR_ARG, "a tile's face record does not fit the LDS of a block");
+  }
   return USG_OK;
 }
 

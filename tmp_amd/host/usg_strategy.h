@@ -149,10 +149,11 @@ class GpuFlowStrategy {
       check(usg_ctx_create(&d, &p, &c));
       ctx_.push_back(c);
       nc_.push_back(d.nc);
+      nf_.push_back(d.nf);
       meshes_.push_back(rd.mesh(i));
     }
-    mirrors_.assign(16, std::vector<std::vector<double>>(ctx_.size()));
-    ptrs_.assign(16, std::vector<double *>(ctx_.size(), nullptr));
+    mirrors_.assign(kFields, std::vector<std::vector<double>>(ctx_.size()));
+    ptrs_.assign(kFields, std::vector<double *>(ctx_.size(), nullptr));
   }
   void setCFL(double lusgs, double rk) {
     cfl_lusgs_ = lusgs;
@@ -176,9 +177,27 @@ class GpuFlowStrategy {
   void preprocessAdvanceSerial(int /*istage*/) {
     for (auto *c : ctx_) check(usg_residual(c));
   }
+  // preprocessAdvance (Euler2D.h:115-131): the MPI variant exchanges halo
+  // cells first; one GPU holds the whole mesh, so it is the serial one
+  void preprocessAdvance(int istage) { preprocessAdvanceSerial(istage); }
+  // postprocessAdvanceSerial (Euler2D.h:142-146) does nothing;
+  // postprocessAdvance (Euler2D.h:165-) exchanges halo cells, which one GPU
+  // does not have
+  void postprocessAdvance(int /*istage*/) {}
+  void postprocessAdvanceSerial(int /*istage*/) {}
+  // SolveTime(CFL) (Euler2D.h:67-77): the local time step is formed inside
+  // each LU-SGS iteration; this sets the CFL number it uses from the next
+  // iteration on
+  void SolveTime(double CFL) {
+    cfl_lusgs_ = CFL;
+    for (auto *c : ctx_) check(usg_set_cfl(c, CFL));
+  }
+  // cleaning (Euler2D.h:150-163) zeroes MPI receive cells; there are none
+  void cleaning(double ** /*W0*/) {}
 
   // getters of TopologyHolderStrategy.h:242-303 (host mirrors, AoS)
   double **getU() { return mirror(USG_F_PRIM, 4); }
+  double **getUEdge() { return mirror(USG_F_UEDGE, 4, true); }
   double **getResidual() { return mirror(USG_F_RES, 4); }
   void **getGradientPrimitive() { return (void **)mirror(USG_F_GRAD, 8); }
   double **getSpectrumConvective() { return mirror(USG_F_LAMC, 1); }
@@ -232,10 +251,11 @@ class GpuFlowStrategy {
   }
 
  private:
-  double **mirror(int field, int width) {
+  static constexpr int kFields = 32;
+  double **mirror(int field, int width, bool faces = false) {
     for (size_t m = 0; m < ctx_.size(); m++) {
       auto &buf = mirrors_[field][m];
-      buf.resize((size_t)width * nc_[m]);
+      buf.resize((size_t)width * (faces ? nf_[m] : nc_[m]));
       check(usg_get_field(ctx_[m], field, buf.data(), (long)buf.size()));
       ptrs_[field][m] = buf.data();
     }
@@ -244,7 +264,7 @@ class GpuFlowStrategy {
   int flow_, precision_;
   double ma_ = 0.5, aoa_ = 0.0, re_ = 1e6, cfl_lusgs_ = 400, cfl_rk_ = 0.8;
   std::vector<usg_ctx *> ctx_;
-  std::vector<int> nc_;
+  std::vector<int> nc_, nf_;
   std::vector<MeshData> meshes_;
   std::vector<std::vector<std::vector<double>>> mirrors_;
   std::vector<std::vector<double *>> ptrs_;

@@ -25,6 +25,17 @@ FP64 state, and FP64 normal and r_A for the LU-SGS terms):
                                          -> Euler 52, laminar/SST 68
   * face outputs: 5 / 10 / 12 floats + 2 doubles -> 36 / 56 / 64
 
+Fused face-cell kernel ``k_facecell`` (FP64: every face of a tile of cells
+into LDS, then each cell's ordered gather), per launch = nc x cell term +
+nf x face term, each input once:
+  * cell inputs: k_face's (144 / 168 / 232) + the cell's face slots 16,
+    volume 8, local face indices 8                  -> 176 / 200 / 264
+  * cell outputs: R 32, Lambda_c 8, dt 8, D 8 (+ Lambda_v 8) (+ the SST
+    residual 16 and diagonal 16)                      -> 56 / 64 / 96
+  * face inputs: k_face's 48 (+ 8 viscous) + the tile face list 4
+  * face outputs: the two LU-SGS r_A values 16 (the face record stays in LDS)
+  so per cell: Euler 232 + 68 r, laminar 264 + 76 r, SST 360 + 76 r.
+
 Whole LU-SGS iteration (the cell-iteration model of the reference path's
 six inherent stages -- gradient, limiter, flux, forward sweep, backward
 sweep, update): Euler 992 + 200 r, laminar 1064 + 200 r, SST 1536 + 200 r.
@@ -52,3 +63,9 @@ def face_kernel_bytes_per_cell(flow: int, r: float, mixed: bool = False) -> floa
 def iteration_bytes_per_cell(flow: int, r: float) -> float:
     base = {0: 992.0, 1: 1064.0, 2: 1536.0}[flow]
     return base + 200.0 * r
+
+
+def fused_kernel_bytes_per_cell(flow: int, r: float) -> float:
+    cell = {0: 176 + 56, 1: 200 + 64, 2: 264 + 96}[flow]
+    face = (48 if flow == 0 else 56) + 4 + 16
+    return cell + face * r

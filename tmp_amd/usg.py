@@ -26,10 +26,12 @@ EULER, LAMINAR, SST = 0, 1, 2
 FP64, MIXED = 0, 1
 F_PRIM, F_CONS, F_RES, F_GRAD, F_LAMC, F_PHI, F_DW1, F_DW = range(8)
 F_LAMV, F_GRADT, F_TURB, F_TRES, F_MUT, F_F1, F_DT, F_TDW, F_TCONS, F_W0 = range(8, 18)
+F_UEDGE, F_WDIST = 18, 19
 K_GRAD, K_FACE, K_CELL, K_SWEEP = range(4)
 
 _FIELD_WIDTH = {F_PRIM: 4, F_CONS: 4, F_RES: 4, F_GRAD: 8, F_LAMC: 1, F_PHI: 4, F_DW1: 4, F_DW: 4,
-                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2, F_TCONS: 2, F_W0: 4}
+                F_LAMV: 1, F_GRADT: 2, F_TURB: 2, F_TRES: 2, F_MUT: 1, F_F1: 1, F_DT: 1, F_TDW: 2, F_TCONS: 2, F_W0: 4,
+                F_WDIST: 1}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -57,7 +59,9 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_device_arch", "usg_case_status", "usg_nodal_weights", "usg_mesh_cells", "usg_freestream_state",
            "usg_write_cell_data", "usg_surface_cp", "usg_write_cp", "usg_set_state", "usg_overset_create",
            "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution",
-           "usg_overset_median", "usg_overset_obb", "usg_stencil_apply_host", "usg_stencil_apply_host_noalloc"]
+           "usg_overset_median", "usg_overset_obb", "usg_stencil_apply_host", "usg_stencil_apply_host_noalloc",
+           "usg_overset_cell_stencil", "usg_set_overset", "usg_dual_time", "usg_dual_time_advance", "usg_set_cfl",
+           "usg_use_fused", "usg_set_order"]
 
 _lib = None
 
@@ -114,6 +118,13 @@ def lib():
         L.usg_stencil_apply_host.argtypes = [_dp, _dp, C.c_int, C.c_int, C.c_long, C.c_long, C.c_int, _ip, _ip, _ip,
                                              _dp]
         L.usg_stencil_apply_host_noalloc.argtypes = L.usg_stencil_apply_host.argtypes
+        L.usg_overset_cell_stencil.argtypes = [vp, C.POINTER(_MeshDesc), C.c_int, _ip, _ip, _ip, _ip, _ip, _ip, _dp]
+        L.usg_set_overset.argtypes = [vp, _ip, C.c_int, _ip, _ip, _ip, _dp]
+        L.usg_dual_time.argtypes = [vp, C.c_double, C.c_int]
+        L.usg_dual_time_advance.argtypes = [vp]
+        L.usg_set_cfl.argtypes = [vp, C.c_double]
+        L.usg_use_fused.argtypes = [vp, C.c_int]
+        L.usg_set_order.argtypes = [vp, C.c_int]
         L.usg_overset_obb.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.usg_last_error.restype = C.c_char_p
         _lib = L
@@ -255,6 +266,55 @@ def overset_connect(meshes, overset_faces=None, mexclude: int = 3, nfringe: int 
     return out
 
 
+def overset_coupling(meshes, overset_faces=None, mexclude: int = 3, nfringe: int = 1) -> tuple[list, dict]:
+    """Connectivity of the blocks (as overset_connect) and the cell-centred
+    coupling stencil of usg_overset_cell_stencil over their cells in block
+    order: {"blank", "recv", "off", "donor", "w"}."""
+    k = len(meshes)
+    descs = (_MeshDesc * k)()
+    keep = []
+    flags = (_ip * k)()
+    for i, m in enumerate(meshes):
+        d, kp = _desc(m)
+        descs[i] = d
+        keep.append(kp)
+        f = None if overset_faces is None or overset_faces[i] is None else \
+            np.ascontiguousarray(overset_faces[i], np.int32)
+        keep.append(f)
+        flags[i] = f.ctypes.data_as(_ip) if f is not None else _ip()
+    L = lib()
+    h = C.c_void_p()
+    _check(L.usg_overset_create(descs, flags, k, int(mexclude), int(nfringe), C.byref(h)))
+    try:
+        conn = []
+        for i, m in enumerate(meshes):
+            nib = np.zeros(m.nn, np.int32)
+            cib = np.zeros(m.nc, np.int32)
+            _check(L.usg_overset_iblank(h, i, nib.ctypes.data_as(_ip), cib.ctypes.data_as(_ip)))
+            n = C.c_int()
+            _check(L.usg_overset_receptors(h, i, C.byref(n), None, None, None, None, None, None))
+            r = {"recv_block": np.zeros(n.value, np.int32), "recv_node": np.zeros(n.value, np.int32),
+                 "donor_cell": np.zeros(n.value, np.int32), "nw": np.zeros(n.value, np.int32),
+                 "donor_node": np.zeros((n.value, 4), np.int32), "w": np.zeros((n.value, 4))}
+            _check(L.usg_overset_receptors(h, i, C.byref(n), *(r[q].ctypes.data_as(_ip) for q in
+                                                               ("recv_block", "recv_node", "donor_cell", "nw",
+                                                                "donor_node")), r["w"].ctypes.data_as(_dp)))
+            conn.append({"node_iblank": nib, "cell_iblank": cib, **r})
+        ntot = sum(m.nc for m in meshes)
+        nr, nd = C.c_int(), C.c_int()
+        blank = np.zeros(ntot, np.int32)
+        _check(L.usg_overset_cell_stencil(h, descs, k, blank.ctypes.data_as(_ip), C.byref(nr), None, None,
+                                          C.byref(nd), None, None))
+        st = {"blank": blank, "recv": np.zeros(nr.value, np.int32), "off": np.zeros(nr.value + 1, np.int32),
+              "donor": np.zeros(nd.value, np.int32), "w": np.zeros(nd.value)}
+        _check(L.usg_overset_cell_stencil(h, descs, k, None, C.byref(nr), st["recv"].ctypes.data_as(_ip),
+                                          st["off"].ctypes.data_as(_ip), C.byref(nd),
+                                          st["donor"].ctypes.data_as(_ip), st["w"].ctypes.data_as(_dp)))
+    finally:
+        L.usg_overset_destroy(h)
+    return conn, st
+
+
 def overset_median(x) -> tuple[np.ndarray, np.ndarray]:
     """find_median_wrap on a copy of x: (partitioned x, companion indices)."""
     x = np.array(x, np.float64)
@@ -365,6 +425,11 @@ class Solver:
         _check(lib().usg_set_state(self._h, p(U, 4), p(W, 4), p(turb, 2), p(tcons, 2), p(mut, 1)))
 
     def field(self, fid: int) -> np.ndarray:
+        if fid == F_UEDGE:
+            nf = sum(m.nf for m in self.meshes)
+            out = np.zeros(nf * 4)
+            _check(lib().usg_get_field(self._h, fid, out.ctypes.data_as(_dp), out.size))
+            return out.reshape(nf, 4)
         w = _FIELD_WIDTH[fid]
         out = np.zeros(self.nc * w)
         _check(lib().usg_get_field(self._h, fid, out.ctypes.data_as(_dp), out.size))
@@ -442,6 +507,10 @@ class Solver:
     def graphs(self, on: bool):
         _check(lib().usg_use_graphs(self._h, int(on)))
 
+    def fused(self, on: bool):
+        """Face fluxes and cell gathers in one launch (FP64; default on)."""
+        _check(lib().usg_use_fused(self._h, int(on)))
+
     def timing(self, on: bool):
         _check(lib().usg_timing_enable(self._h, int(on)))
 
@@ -491,6 +560,33 @@ class Solver:
         with open(path, "wb") as f:
             np.savez(f, **d)
 
+
+    # --- overset coupling and dual time ---------------------------------
+    def set_overset(self, st: dict | None) -> None:
+        """usg_set_overset with a stencil of overset_coupling (None removes it)."""
+        L = lib()
+        if st is None:
+            _check(L.usg_set_overset(self._h, None, 0, None, None, None, None))
+            return
+        arrs = [np.ascontiguousarray(st[q], np.int32) for q in ("blank", "recv", "off", "donor")]
+        w = np.ascontiguousarray(st["w"], np.float64)
+        _check(L.usg_set_overset(self._h, arrs[0].ctypes.data_as(_ip), arrs[1].size, arrs[1].ctypes.data_as(_ip),
+                                 arrs[2].ctypes.data_as(_ip), arrs[3].ctypes.data_as(_ip), w.ctypes.data_as(_dp)))
+
+    def set_cfl(self, cfl: float) -> None:
+        """LU-SGS pseudo-time CFL number from the next iteration on."""
+        _check(lib().usg_set_cfl(self._h, float(cfl)))
+        self.cfl = (float(cfl), self.cfl[1])
+
+    def set_order(self, order: int) -> None:
+        """Reconstruction order from the next residual on (2: limited linear, 1: cell values)."""
+        _check(lib().usg_set_order(self._h, int(order)))
+
+    def dual_time(self, dt_phys: float, order: int = 2) -> None:
+        _check(lib().usg_dual_time(self._h, float(dt_phys), int(order)))
+
+    def dual_time_advance(self) -> None:
+        _check(lib().usg_dual_time_advance(self._h))
 
     # --- launch configuration -------------------------------------------
     def blocks(self) -> list:
