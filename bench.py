@@ -233,7 +233,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed: FP32 gradients/limiter/fluxes, FP64 state, accumulation, diagonal and norms")
     ap.add_argument("--autotune", action="store_true",
-                    help="pick threads per block per kernel group (cached in ~/.cache/tmp_amd/autotune.json)")
+                    help="pick threads per block per kernel group and the face-cell launch scheme "
+                         "(cached in ~/.cache/tmp_amd/autotune.json)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (traffic-counter calibration)")
     a = ap.parse_args()
@@ -290,10 +291,12 @@ def main():
     if a.autotune:
         from tmp_amd import autotune
         t = time.perf_counter()
-        blocks, hit = autotune.tune(solver, prec, nf)
-        print(f"# autotune {'cache hit' if hit else 'search'}: blocks {blocks} ({time.perf_counter() - t:.1f}s)",
+        cfg, hit = autotune.tune(solver, prec, nf)
+        print(f"# autotune {'cache hit' if hit else 'search'}: launch config {cfg} ({time.perf_counter() - t:.1f}s)",
               file=sys.stderr, flush=True)
     config["threads_per_block"] = solver.blocks()
+    fused = solver.is_fused()
+    config["face_cell_launch"] = "fused" if fused else "separate"
     if a.warmup:
         solver.lusgs(a.warmup)
     solver.sync()
@@ -324,9 +327,9 @@ def main():
               flush=True)
     value = nc * a.steps / elapsed
     r = nf / nc
-    # the dominant kernel: the fused face-cell launch in FP64, the face launch otherwise
-    kname = "k_facecell" if not mixed else "k_face"
-    kbytes = (roofline.fused_kernel_bytes_per_cell(flow, r) if not mixed
+    # the dominant kernel: the face launch (or the fused face-cell launch)
+    kname = "k_facecell" if fused else "k_face"
+    kbytes = (roofline.fused_kernel_bytes_per_cell(flow, r) if fused
               else roofline.face_kernel_bytes_per_cell(flow, r, mixed)) * nc
     kavg_s = ktime_ms / 1e3 / max(klaunch, 1)
     achieved = kbytes / kavg_s / 1e9

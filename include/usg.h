@@ -195,13 +195,17 @@ int usg_mesh_renumber(const usg_mesh_desc *mesh, int *perm, int *fnode, int *flc
 /* LU-SGS iterations run as replays of a captured two-iteration hipGraph
  * (default on); off = eager launches.  Results are identical either way. */
 int usg_use_graphs(usg_ctx *ctx, int on);
-/* FP64 residuals compute the face fluxes and gather them into the cells in
- * one launch, the face record held in LDS per tile of cells (default on);
- * off = a face launch that writes the face record to HBM and a cell launch
- * that reads it back.  Results are identical either way.  Fails with
- * USG_ERR_ARG (and stays off) when a tile's face record would not fit the
- * LDS of a block.  No counterpart in the reference (a launch choice). */
+/* FP64 residuals: on = the face fluxes and the cell gathers in one launch,
+ * the face record held in LDS per tile of cells; off (default) = a face
+ * launch that writes the face record to HBM and a cell launch that reads it
+ * back.  Results are identical either way; which is faster depends on the
+ * mesh (the fused launch computes the faces between tiles twice and holds
+ * fewer waves per CU), so the autotuner picks it.  Fails with USG_ERR_ARG
+ * (and stays off) when a tile's face record would not fit the LDS of a
+ * block.  usg_get_fused reports the scheme in use (0 in mixed precision).
+ * No counterpart in the reference (a launch choice). */
 int usg_use_fused(usg_ctx *ctx, int on);
+int usg_get_fused(usg_ctx *ctx, int *on);
 
 /* Threads per block of each kernel group (64, 128 or 256; default 256):
  * the gradient/limiter kernel, the face flux kernel, the cell gather kernel
@@ -214,9 +218,12 @@ int usg_use_fused(usg_ctx *ctx, int on);
 int usg_set_block(usg_ctx *ctx, int group, int threads);
 int usg_get_block(usg_ctx *ctx, int group, int *threads);
 /* Autotuner: for each group in turn, times `iters` eager LU-SGS iterations
- * with 64, 128 and 256 threads per block and keeps the fastest; the state
- * is restored afterwards, so tuning does not change the solution.  best[4]
- * (may be NULL) receives the chosen sizes.  No counterpart in the reference. */
+ * with 64, 128 and 256 threads per block and keeps the fastest; in FP64 the
+ * cell group also tries the fused face-cell launch (usg_use_fused) at each
+ * size, whose face tile is the cell block.  The state is restored
+ * afterwards, so tuning does not change the solution.  best[4] (may be
+ * NULL) receives the chosen sizes; the chosen scheme stays set
+ * (usg_get_fused).  No counterpart in the reference. */
 int usg_autotune(usg_ctx *ctx, int iters, int *best);
 /* gcnArchName of the current device (key of autotuning caches). */
 int usg_device_arch(char *buf, int n);

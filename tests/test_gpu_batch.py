@@ -74,17 +74,18 @@ def test_batch_rk5_equals_single_cases():
 
 
 def test_autotune_restores_state_and_blocks_do_not_change_results():
-    """Tuning leaves the state as it found it, and any block sizes give the
-    same bits (one thread per cell or face whatever the block size)."""
+    """Tuning leaves the state as it found it, and any block sizes and
+    either face-cell launch scheme give the same bits (one thread per face,
+    four lanes or one thread per cell, whatever the block size)."""
     m = airfoil_small()
     a = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
     b = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
     a.lusgs(3)
     b.lusgs(3)
     best = a.autotune(2)
-    assert all(t in (64, 128, 256) for t in best) and a.blocks() == best
+    assert all(t in (64, 128, 256) for t in best[:4]) and best[4] in (0, 1) and a.launch_config() == best
     assert np.array_equal(a.U, b.U)
-    a.set_blocks([64, 128, 64, 128])
+    a.set_launch_config([64, 128, 64, 128, 1 - best[4]])
     ha = a.lusgs(5, history=True)
     hb = b.lusgs(5, history=True)
     assert np.array_equal(ha, hb) and np.array_equal(a.U, b.U)

@@ -1,15 +1,17 @@
-"""Launch-size autotuner with a persistent cache.
+"""Launch autotuner with a persistent cache.
 
-``tune(solver)`` picks the threads per block of the four kernel groups
-(gradient/limiter, face flux, cell gather, sweep waves) for a solver's
-problem shape: it looks the shape up in a JSON cache and, on a miss, runs
-the library's timed search (``usg_autotune``: coordinate descent over 64,
-128 and 256 threads, the state restored afterwards) and stores the result.
-In FP64 the face fluxes and cell gathers run as one fused launch whose
-tiles are the cell group's blocks, so the cell group's size is the face
-tile: 64, 128 or 256 cells and every face they touch.
-The key is (GPU arch, the launch scheme, flow type, precision, number of cases, cells rounded
-to 2 significant digits, faces/cells ratio rounded to 0.1).
+``tune(solver)`` picks the launch configuration for a solver's problem
+shape: the threads per block of the four kernel groups (gradient/limiter,
+face flux, cell gather, sweep waves) and, in FP64, whether the face fluxes
+and cell gathers run as two launches or as one fused launch whose face tile
+is the cell group's block (64, 128 or 256 cells and every face they touch).
+It looks the shape up in a JSON cache and, on a miss, runs the library's
+timed search (``usg_autotune``: coordinate descent over 64, 128 and 256
+threads, both schemes for the cell group, the state restored afterwards)
+and stores the result as ``Solver.launch_config()``.
+The key is (GPU arch, "launch-scheme", flow type, precision, number of
+cases, cells rounded to 2 significant digits, faces/cells ratio rounded to
+0.1).
 """
 from __future__ import annotations
 
@@ -37,7 +39,7 @@ def _round2(n: int) -> int:
 
 
 def shape_key(solver, precision: int, nf: int) -> str:
-    return "|".join([device_arch(), "fused-tiles", str(solver.flow), str(precision), str(solver.ncase),
+    return "|".join([device_arch(), "launch-scheme", str(solver.flow), str(precision), str(solver.ncase),
                      str(_round2(solver.nc)), f"{nf / solver.nc:.1f}"])
 
 
@@ -51,11 +53,12 @@ def load(path: str) -> dict:
 
 def tune(solver, precision: int, nf: int, iters: int = 5, cache: str | None = DEFAULT_CACHE,
          force: bool = False) -> tuple[list, bool]:
-    """Apply the cached (or freshly searched) block sizes; returns (blocks, hit)."""
+    """Apply the cached (or freshly searched) launch configuration; returns
+    (config, hit)."""
     key = shape_key(solver, precision, nf)
     table = load(cache) if cache else {}
     if not force and key in table:
-        solver.set_blocks(table[key])
+        solver.set_launch_config(table[key])
         return table[key], True
     best = solver.autotune(iters)
     if cache:

@@ -61,7 +61,7 @@ EXPORTS = ["usg_ctx_create", "usg_ctx_destroy", "usg_set_stream", "usg_sync", "u
            "usg_overset_destroy", "usg_overset_iblank", "usg_overset_receptors", "usg_overset_resolution",
            "usg_overset_median", "usg_overset_obb", "usg_stencil_apply_host", "usg_stencil_apply_host_noalloc",
            "usg_overset_cell_stencil", "usg_set_overset", "usg_dual_time", "usg_dual_time_advance", "usg_set_cfl",
-           "usg_use_fused", "usg_set_order"]
+           "usg_use_fused", "usg_get_fused", "usg_set_order"]
 
 _lib = None
 
@@ -124,6 +124,7 @@ def lib():
         L.usg_dual_time_advance.argtypes = [vp]
         L.usg_set_cfl.argtypes = [vp, C.c_double]
         L.usg_use_fused.argtypes = [vp, C.c_int]
+        L.usg_get_fused.argtypes = [vp, _ip]
         L.usg_set_order.argtypes = [vp, C.c_int]
         L.usg_overset_obb.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
         L.usg_last_error.restype = C.c_char_p
@@ -508,8 +509,13 @@ class Solver:
         _check(lib().usg_use_graphs(self._h, int(on)))
 
     def fused(self, on: bool):
-        """Face fluxes and cell gathers in one launch (FP64; default on)."""
+        """Face fluxes and cell gathers in one launch (FP64; default off)."""
         _check(lib().usg_use_fused(self._h, int(on)))
+
+    def is_fused(self) -> bool:
+        v = C.c_int()
+        _check(lib().usg_get_fused(self._h, C.byref(v)))
+        return bool(v.value)
 
     def timing(self, on: bool):
         _check(lib().usg_timing_enable(self._h, int(on)))
@@ -601,10 +607,21 @@ class Solver:
         for g, t in enumerate(blocks):
             _check(lib().usg_set_block(self._h, g, int(t)))
 
+    def launch_config(self) -> list:
+        """[threads per block of the gradient, face, cell and sweep groups,
+        1 if the fused face-cell launch is in use else 0]."""
+        return self.blocks() + [int(self.is_fused())]
+
+    def set_launch_config(self, cfg) -> None:
+        self.set_blocks(cfg[:4])
+        if len(cfg) > 4:
+            self.fused(bool(cfg[4]))
+
     def autotune(self, iters: int = 5) -> list:
-        best = np.zeros(4, np.int32)
-        _check(lib().usg_autotune(self._h, iters, best.ctypes.data_as(_ip)))
-        return best.tolist()
+        """Runs usg_autotune (block sizes and, in FP64, the launch scheme);
+        returns the chosen launch_config()."""
+        _check(lib().usg_autotune(self._h, iters, None))
+        return self.launch_config()
 
 
 def load_checkpoint(path: str, precision: int | None = None) -> "Solver":
