@@ -75,8 +75,8 @@ def test_batch_rk5_equals_single_cases():
 
 def test_autotune_restores_state_and_blocks_do_not_change_results():
     """Tuning leaves the state as it found it, and any block sizes and
-    either face-cell launch scheme give the same bits (one thread per face,
-    four lanes or one thread per cell, whatever the block size)."""
+    either face-cell launch scheme give the same bits (one thread per face
+    or cell, whatever the block size)."""
     m = airfoil_small()
     a = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)
     b = usg.Solver(m, flow=usg.SST, mach=0.15, aoa=4.0, reynolds=6e6)

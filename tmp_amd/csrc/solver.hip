@@ -313,179 +313,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   }
 }
 
-// ---------------------------------------------------------------------------
-// k_grad4: k_grad's FP64 work with four lanes per cell, lane q on face slot q
-// (the cell's 4th slot is empty for a triangle).  Each lane gathers its
-// neighbour and computes its face's terms; the cell's sums are then formed
-// in every lane of the group, in slot order, from the four lanes' terms
-// (quad broadcasts, no LDS), so they equal k_grad's serial slot loop bit
-// for bit: g + x over the present slots, x negated for a right-cell face
-// (a - y is a + (-y) in IEEE arithmetic).  One slot's registers per lane
-// instead of four lets more waves hide the gathers' latency.
-// ---------------------------------------------------------------------------
-// lane K's value within each group of four lanes (DPP quad_perm broadcast);
-// every lane of the group must be active
-template <int K>
-__device__ __forceinline__ double qlane(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), K * 0x55, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), K * 0x55, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-template <int K>
-__device__ __forceinline__ int qlane(int v) {
-  return __builtin_amdgcn_update_dpp(0, v, K * 0x55, 0xF, 0xF, false);
-}
-// ((0 + x_0) + x_1) + ... over the n present slots, as the serial loop adds
-__device__ __forceinline__ double qsum(double x, int n) {
-  const double x0 = qlane<0>(x), x1 = qlane<1>(x), x2 = qlane<2>(x), x3 = qlane<3>(x);
-  double a = 0.0;
-  if (n > 0) a = a + x0;
-  if (n > 1) a = a + x1;
-  if (n > 2) a = a + x2;
-  if (n > 3) a = a + x3;
-  return a;
-}
-
-template <int FLOW, bool B>
-__global__ __launch_bounds__(kBlock) void k_grad4(const DevMesh m, const DevState s, const Consts k_,
-                                                 const int *__restrict__ items, const XSlice sl) {
-  using F = Flow<FLOW>;
-  const int part = blockIdx.x % kXcd;
-  const int i = sl.off[part] + (int)(blockIdx.x / kXcd) * ((int)blockDim.x >> 2) + ((int)threadIdx.x >> 2);
-  if (i >= sl.off[part + 1]) return;  // the whole group of four leaves together
-  const int q = (int)threadIdx.x & 3;
-  const int c = items[i];
-  const Consts k = case_consts<B>(m, k_, c);
-  const int nc = m.nc;
-  double Uc[4];
-  load4(s.U, nc, c, Uc);
-  const double V = m.vol[c];
-  double tUc[2] = {0, 0};
-  if (F::sst) {
-    tUc[0] = s.tU[c];
-    tUc[1] = s.tU[nc + c];
-  }
-  // this lane's slot
-  Slot o;
-  const bool has = get_slot(m, q, c, o);
-  const int n = qlane<0>((int)has) + qlane<1>((int)has) + qlane<2>((int)has) + qlane<3>((int)has);
-  double x[8] = {0, 0, 0, 0, 0, 0, 0, 0}, xT[2] = {0, 0}, xt[4] = {0, 0, 0, 0};
-  double Unb[4] = {Uc[0], Uc[1], Uc[2], Uc[3]};  // interior neighbour (else the cell: max/min unchanged)
-  double v0 = 0, v1 = 0, Vl = V;
-  int bad = 0;
-  if (has) {
-    const double n0 = m.fnx[o.f], n1 = m.fny[o.f], A = m.fA[o.f];
-    const double sg = o.left ? 1.0 : -1.0;
-    double Ue[4];
-    if (o.nb >= 0) {
-      load4(s.U, nc, o.nb, Unb);
-      for (int j = 0; j < 4; j++) Ue[j] = o.left ? (Uc[j] + Unb[j]) / 2 : (Unb[j] + Uc[j]) / 2;
-    } else if (!boundary_state<F::visc>(k, o.nb, Uc, n0, n1, Ue)) {
-      bad = 1;
-    }
-    for (int j = 0; j < 4; j++) {
-      x[2 * j] = sg * (n0 * (A * Ue[j]));
-      x[2 * j + 1] = sg * (n1 * (A * Ue[j]));
-    }
-    if (F::visc) {
-      const double edgeT = Ue[1] / Ue[0];
-      xT[0] = sg * (n0 * (A * edgeT));
-      xT[1] = sg * (n1 * (A * edgeT));
-      if (F::sst) {
-        // ComputeFaceTValue (SSTKWStrategy.h:616-659)
-        double te[2];
-        if (o.nb >= 0) {
-          const double t0 = s.tU[o.nb], t1 = s.tU[nc + o.nb];
-          te[0] = o.left ? 0.5 * (tUc[0] + t0) : 0.5 * (t0 + tUc[0]);
-          te[1] = o.left ? 0.5 * (tUc[1] + t1) : 0.5 * (t1 + tUc[1]);
-        } else if (o.nb == kWallD) {
-          const double dis = m.wdist[c];
-          te[0] = 0.0;
-          te[1] = 10.0 * 6.0 * sutherland<double>(k, edgeT) / (Ue[0] * SST_BETA1 * dis * dis);
-        } else if (dot2(Ue[2], Ue[3], n0, n1) > 0.0) {
-          te[0] = tUc[0];
-          te[1] = tUc[1];
-        } else {
-          te[0] = k.fsnd_turb[0];
-          te[1] = k.fsnd_turb[1];
-        }
-        for (int t = 0; t < 2; t++) {
-          xt[2 * t] = sg * (n0 * (A * te[t]));
-          xt[2 * t + 1] = sg * (n1 * (A * te[t]));
-        }
-      }
-    }
-    face_offset<double>(m, c, o.f, o.left, v0, v1);
-    if (!o.left) Vl = m.vol[o.nb];
-  }
-  // the cell's sums, in every lane
-  double g[8];
-  for (int j = 0; j < 8; j++) g[j] = qsum(x[j], n) / V;
-  // Umax/Umin over the cell and its interior neighbours, in slot order
-  double umax[4], umin[4];
-  for (int j = 0; j < 4; j++) {
-    const double u0 = qlane<0>(Unb[j]), u1 = qlane<1>(Unb[j]), u2 = qlane<2>(Unb[j]), u3 = qlane<3>(Unb[j]);
-    umax[j] = smax(smax(smax(smax(Uc[j], u0), u1), u2), u3);
-    umin[j] = smin(smin(smin(smin(Uc[j], u0), u1), u2), u3);
-  }
-  double *G = s.grad;
-  G[(2 * q) * nc + c] = g[2 * q];
-  G[(2 * q + 1) * nc + c] = g[2 * q + 1];
-  double muc = 0;
-  if (F::visc) {
-    const double gT0 = qsum(xT[0], n) / V, gT1 = qsum(xT[1], n) / V;
-    if (q == 0) s.gradT[c] = gT0;
-    if (q == 1) s.gradT[nc + c] = gT1;
-    muc = sutherland<double>(k, Uc[1] / Uc[0]);
-    if (q == 0) s.mu[c] = muc;
-  }
-  bad = qlane<0>(bad) | qlane<1>(bad) | qlane<2>(bad) | qlane<3>(bad);
-  if (bad && q == 0) {
-    atomicAdd(&s.err[1], 1);
-    if constexpr (B) atomicOr(&s.err[2 + m.ccase[c]], 2);
-  }
-  // Venkatakrishnan limiter: this lane's slot, then the minimum in slot order
-  // (eps^2 = 5^3 * V_left, integer 3/ndim == 1)
-  double ven[4] = {1.0, 1.0, 1.0, 1.0};
-  if (has) {
-    const double eps2 = 125.0 * Vl;
-    for (int j = 0; j < 4; j++)
-      ven[j] = venkat(umax[j] - Uc[j], umin[j] - Uc[j], dot2(g[2 * j], g[2 * j + 1], v0, v1), eps2);
-  }
-  double phq = 1.0;
-  for (int j = 0; j < 4; j++) {
-    const double a0 = qlane<0>(ven[j]), a1 = qlane<1>(ven[j]), a2 = qlane<2>(ven[j]), a3 = qlane<3>(ven[j]);
-    double ph = 1.0;
-    if (n > 0) ph = smin(ph, a0);
-    if (n > 1) ph = smin(ph, a1);
-    if (n > 2) ph = smin(ph, a2);
-    if (n > 3) ph = smin(ph, a3);
-    if (j == q) phq = ph;
-  }
-  if (k.order == 1) phq = 0.0;  // first order: the face states are the cell values
-  s.phi[q * nc + c] = phq;
-
-  if (F::sst) {
-    double tg[4];
-    for (int t = 0; t < 4; t++) tg[t] = qsum(xt[t], n) / V;
-    s.tgrad[q * nc + c] = tg[q];
-    // ComputeSSTf1f2 (SSTKWStrategy.h:695-722): F1 in lane 0, F2 in lane 1
-    const double gradVal = dot2(tg[0], tg[1], tg[2], tg[3]);
-    const double rho = Uc[0], dis = m.wdist[c];
-    const double Kk = tUc[0], w = tUc[1];
-    const double muL = muc;
-    if (q == 0) {
-      const double CD_kw = smax(2.0 * rho * SST_SW2 / w * gradVal, 1.0e-20);
-      double arg1 = smax(sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
-      arg1 = smin(arg1, 4.0 * rho * SST_SW2 * Kk / (CD_kw * dis * dis));
-      s.F1[c] = tanh(arg1 * arg1 * arg1 * arg1);
-    } else if (q == 1) {
-      const double arg2 = smax(2.0 * sqrt(Kk) / (0.09 * w * dis), 500.0 * muL / (rho * w * dis * dis));
-      s.F2[c] = tanh(arg2 * arg2);
-    }
-  }
-}
-
 // Limited reconstruction of one side: UL = (grad . (x_f - x_c)) * phi + U
 // (Vankatakrishnan_Limiter.hpp:118-125, HLLCFluxStrategy.h:56-65).
 template <typename T>
@@ -1684,12 +1511,8 @@ template <int FLOW, typename T, bool B>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
   const int bg = c->blk[USG_K_GRAD], bf = c->blk[USG_K_FACE], bc = c->blk[USG_K_CELL];
-  if constexpr (std::is_same<T, double>::value)
-    hipLaunchKernelGGL((k_grad4<FLOW, B>), dim3(xgrid(cs, bg / 4)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_cell_list, cs);
-  else
-    hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs, bg)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
-                       c->d_cell_list, cs);
+  hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs, bg)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
+                     c->d_cell_list, cs);
   if constexpr (std::is_same<T, double>::value) {
     if (fused_ready(c)) {
       if (ea) hipEventRecord(ea, c->stream);
