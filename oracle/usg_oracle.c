@@ -697,9 +697,16 @@ static void compute_limiter(orc_solver *s) {
 /* HLLCFluxStrategy::updateLeftRight (HLLCFluxStrategy.h:35-100) */
 static void update_left_right(orc_solver *s) {
   compute_limiter(s);
-  if (s->order == 1)
-    for (int k = 0; k < s->m->nc * NE; k++) s->phi[k] = 0.0;
   const orc_mesh *m = s->m;
+  if (s->order == 1)
+    for (int k = 0; k < m->nc * NE; k++) s->phi[k] = 0.0;
+  /* overset hole and receptor cells reconstruct first order (no counterpart
+   * in the reference): a receptor's Green-Gauss gradient reaches into the
+   * hole cells beside it, which hold no solution */
+  if (s->blank)
+    for (int c = 0; c < m->nc; c++)
+      if (s->blank[c] != 1)
+        for (int j = 0; j < NE; j++) s->phi[j + NE * c] = 0.0;
   for (int k = 0; k < m->nf; k++) {
     int lc = m->flc[k], rc = m->frc[k];
     if (lc >= 0 && rc >= 0) {

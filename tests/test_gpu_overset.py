@@ -94,6 +94,9 @@ def test_c2_blanked_cells_keep_their_state():
     hole = st["blank"] == 0
     assert np.array_equal(U[hole], U0[hole])
     assert np.any(U[st["blank"] == 1] != U0[st["blank"] == 1])
+    gpu.residual()
+    ph = gpu.field(usg.F_PHI)
+    assert np.all(ph[st["blank"] != 1] == 0.0) and np.all(ph[st["blank"] == 1] > 0.0)
 
 
 def test_dual_time_single_mesh_against_oracle():

@@ -92,7 +92,9 @@ def test_dual_time_term_closed_form(order):
 
 def test_overset_cells_frozen_in_the_sweeps():
     """Hole and receptor cells take no increment and keep their state in an
-    LU-SGS iteration; the norm sums field cells only."""
+    LU-SGS iteration and reconstruct first order (limiter 0, so a
+    receptor's gradient, which reaches into hole cells, never enters a face
+    state); the norm sums field cells only."""
     og, bg, conn, st = c2()
     s = O.OracleSolver(O.OracleMesh(bg), O.EULER, 0.3, 0.0, 1e6)
     blank = st["blank"][og.nc:]
@@ -102,6 +104,8 @@ def test_overset_cells_frozen_in_the_sweeps():
     s.set_blank(blank)
     s.lusgs_step()
     fz = blank != 1
+    ph = s.field("phi", 4)
+    assert np.all(ph[fz] == 0.0) and np.all(ph[~fz] > 0.0)
     assert np.all(s.field("dW", 4)[fz] == 0.0) and np.array_equal(s.U[fz], U0[fz])
     assert np.any(s.U[~fz] != U0[~fz])
     r = s.R[:, 0] / s.mesh.vol

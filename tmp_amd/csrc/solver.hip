@@ -290,8 +290,10 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
       ph[j] = smin(ph[j], venkat(d1max[j], d1min[j], delta2, eps2));
     }
   }
-  if (k.order == 1)
-    for (int j = 0; j < 4; j++) ph[j] = 0.0;  // first order: the face states are the cell values
+  // first order (and overset hole and receptor cells, whose gradients reach
+  // into hole cells that hold no solution): the face states are the cell values
+  if (k.order == 1 || frozen_cell(m, c))
+    for (int j = 0; j < 4; j++) ph[j] = 0.0;
   T *P = pick<T>(s.phi, s.phif);
   for (int j = 0; j < 4; j++) P[j * nc + c] = ph[j];
 
@@ -361,9 +363,9 @@ __device__ __forceinline__ double lusgs_ra(const double *Ua, double n0, double n
 // spectral term, 5-8 the viscous flux, 9 the viscous spectral term, 10-11
 // the SST flux, all area-weighted and oriented out of the left cell; the
 // sweeps' r_A of both sides go to their cell slots.
-// The single-case FP64 variants are held to 96 VGPRs (5 waves per SIMD
-// instead of 4; they fit without spilling and run ~2% faster on the 1 M-cell
-// SST mesh); the batched FP64 variants would spill and keep the default.
+// No occupancy attribute: held to 96 VGPRs (5 waves per SIMD) the SST face
+// kernel spills 10 VGPRs and runs 198 us instead of 179 us per launch on the
+// 1 M-cell mesh; at its own 110 VGPRs it runs 4 waves without spilling.
 // ---------------------------------------------------------------------------
 template <int FLOW, typename T, bool B, typename Out>
 __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, const Consts &k_, int f, Out out) {
@@ -571,8 +573,8 @@ struct FaceOutGlobal {
 };
 
 template <int FLOW, typename T, bool B>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(std::is_same<T, double>::value && !B ? 5 : 1)))
-void k_face(const DevMesh m, const DevState s, const Consts k_, const int *__restrict__ items, const XSlice sl) {
+__global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState s, const Consts k_,
+                                                const int *__restrict__ items, const XSlice sl) {
   int f;
   if (!xcd_item(items, sl, f)) return;
   face_terms<FLOW, T, B>(m, s, k_, f, FaceOutGlobal<T>{pick<T>(s.FF, s.FFf) + f, m.nf});

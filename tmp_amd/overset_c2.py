@@ -29,9 +29,12 @@ MACH, RE = 0.2, 100.0
 
 
 def build(ni: int = 256, nj: int = 96, nx: int = 400, ny: int = 200):
-    """The two grids, their overset-face flags and the coupling stencil."""
-    og, flags = mesh.cylinder_ogrid(ni=ni, nj=nj)
-    bg = mesh.background(nx=nx, ny=ny)
+    """The two grids (colour-blocked numbering, so that each LU-SGS sweep
+    is a few waves instead of one level per diagonal of the structured
+    grid), their connectivity and the coupling stencil."""
+    og = usg.mesh_renumber(mesh.cylinder_ogrid(ni=ni, nj=nj)[0])[0]
+    bg = usg.mesh_renumber(mesh.background(nx=nx, ny=ny))[0]
+    flags = (og.frc == mesh.FARFIELD).astype(np.int32)   # the O-grid's outer boundary is its overset boundary
     conn, st = usg.overset_coupling([og, bg], [flags, None])
     return og, bg, conn, st
 
