@@ -23,6 +23,9 @@ WORKLOADS = {
     "sst1m": ("airfoil_1m", 2, 0.15, 4.0, 6e6, "SST k-omega RANS FP64, NACA0012 hybrid ~1M cells"),
     "euler1m": ("airfoil_1m", 0, 0.15, 4.0, 6e6, "Euler FP64, NACA0012 hybrid ~1M cells"),
     "bump20k": ("bump", 0, 0.5, 0.0, 1e6, "Euler FP64, channel bump ~20k cells"),
+    # node valences 3-8: the gathers of an irregular triangulation
+    "tri1m": ("cylinder_delaunay", 2, 0.15, 0.0, 6e6,
+              "SST k-omega RANS FP64, cylinder, irregular Delaunay triangulation ~1M cells"),
     # 64 NACA 4-digit cases (seed 4004), per-case Mach and alpha, one batched context
     "batch64": ("batch", 2, None, None, 6e6, "SST k-omega RANS FP64, 64-case NACA 4-digit batch (seed 4004)"),
     # cylinder O-grid over a Cartesian background: overset coupling, dual time (BDF2, dt 0.2)
@@ -37,6 +40,8 @@ def build_mesh(kind):
         m = mesh.airfoil_1m()
     elif kind == "bump":
         m = mesh.channel_bump(seed=1001)
+    elif kind == "cylinder_delaunay":
+        m = mesh.cylinder_delaunay()
     else:
         raise ValueError(kind)
     return mesh.prepare(m)
@@ -235,6 +240,10 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="pick threads per block per kernel group and the face-cell launch scheme "
                          "(cached in ~/.cache/tmp_amd/autotune.json)")
+    ap.add_argument("--face-cell", default="auto", choices=["auto", "separate", "fused"],
+                    help="face-cell launch scheme (FP64): the library's default or the autotuner's choice (auto), "
+                         "a face launch and a cell launch through HBM (separate), or one launch per tile with the "
+                         "face record in LDS (fused)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run the plain copy kernel over MB megabytes (traffic-counter calibration)")
     a = ap.parse_args()
@@ -294,6 +303,8 @@ def main():
         cfg, hit = autotune.tune(solver, prec, nf)
         print(f"# autotune {'cache hit' if hit else 'search'}: launch config {cfg} ({time.perf_counter() - t:.1f}s)",
               file=sys.stderr, flush=True)
+    if a.face_cell != "auto":
+        solver.fused(a.face_cell == "fused")
     config["threads_per_block"] = solver.blocks()
     fused = solver.is_fused()
     config["face_cell_launch"] = "fused" if fused else "separate"

@@ -9,7 +9,6 @@ import numpy as np
 from tmp_amd import mesh
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-REFERENCE = "/root/reference"
 
 
 @functools.lru_cache(maxsize=None)
@@ -30,6 +29,13 @@ def bump():
 def airfoil_small():
     """Hybrid NACA 0012 O-mesh, ~8k cells, colour-blocked numbering."""
     return mesh.prepare(mesh.airfoil_hybrid(ni=64, dy1=1e-4, rfar=20.0))[0]
+
+
+@functools.lru_cache(maxsize=None)
+def delaunay_small():
+    """Irregular Delaunay triangulation around a cylinder (node valences
+    3-8), ~8k cells, colour-blocked numbering."""
+    return mesh.prepare(mesh.cylinder_delaunay(8000, r_far=15.0))[0]
 
 
 def two_cells() -> mesh.Mesh:

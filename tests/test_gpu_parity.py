@@ -13,7 +13,7 @@ import pytest
 from oracle import oracle as O
 from tmp_amd import usg
 
-from .common import airfoil_small, bump, part2, perturbed, relerr
+from .common import airfoil_small, bump, delaunay_small, part2, perturbed, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -38,6 +38,7 @@ CASES = [
     ("bump", bump, O.EULER, 0.5, 0.0, 1e6),
     ("airfoil-laminar", airfoil_small, O.LAMINAR, 0.5, 2.0, 5000.0),
     ("airfoil-sst", airfoil_small, O.SST, 0.15, 4.0, 6e6),
+    ("delaunay-sst", delaunay_small, O.SST, 0.15, 0.0, 6e6),
 ]
 
 
@@ -71,7 +72,9 @@ def test_residual_parity(name, mk, flow, mach, aoa, re):
     ("part2-euler", part2, O.EULER, 0.5, 2.0, 1e6, 40),
     ("airfoil-laminar", airfoil_small, O.LAMINAR, 0.5, 2.0, 5000.0, 30),
     ("airfoil-sst", airfoil_small, O.SST, 0.15, 4.0, 6e6, 20),
-], ids=["bump-euler", "part2-euler", "airfoil-laminar", "airfoil-sst"])
+    ("delaunay-laminar", delaunay_small, O.LAMINAR, 0.2, 0.0, 100.0, 30),
+    ("delaunay-sst", delaunay_small, O.SST, 0.15, 0.0, 6e6, 20),
+], ids=["bump-euler", "part2-euler", "airfoil-laminar", "airfoil-sst", "delaunay-laminar", "delaunay-sst"])
 def test_lusgs_trajectory(name, mk, flow, mach, aoa, re, n):
     """n LU-SGS iterations from the free stream: GPU wave-scheduled sweeps vs
     the oracle's sequential sweeps (LUSGS_Integrator.cpp:492-561)."""

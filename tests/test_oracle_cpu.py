@@ -13,7 +13,7 @@ import pytest
 from oracle import oracle as O
 from tmp_amd import mesh
 
-from .common import REFERENCE, bump, part2, perturbed
+from .common import GOLDEN, bump, part2, perturbed
 
 G = 1.4
 
@@ -71,12 +71,10 @@ def test_product_geometry_bit_exact():
         assert (g["nfwd"], g["nbwd"]) == (nfl, nbl)
 
 
-@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE, "mix_element_laminar.grd")),
-                    reason="reference fixtures not mounted")
 def test_reference_fixtures_shape_and_levels():
     """Counts of the shipped meshes and their LU-SGS DAG depth in file
     numbering: 529 levels for mix_element_laminar.grd, 122 for part2.grd."""
-    mx = mesh.read_grd(os.path.join(REFERENCE, "mix_element_laminar.grd"), -1, -2)
+    mx = mesh.read_grd(os.path.join(GOLDEN, "mix_element_laminar.grd.gz"), -1, -2)
     assert (mx.nn, mx.nf, mx.nc) == (40359, 80308, 39949)
     assert (mx.frc == mesh.WALL).sum() == 199 and (mx.frc == mesh.FARFIELD).sum() == 621
     assert O.OracleMesh(mx).levels()[2] == 529

@@ -9,6 +9,7 @@
   cells; a linear field is reproduced exactly where the donor mesh is the
   uniform background (the node means of a uniform grid and the vertex mean
   of a cell are exact for linear fields).
+* The same on the reference's own pair (part1 + part2, test.input).
 * The dual-time term of the oracle's residual: with the free stream as the
   current state (zero spatial residual) and earlier time levels on a
   straight line in time, W^n = W - a dt and W^(n-1) = W - 2 a dt, the
@@ -44,6 +45,47 @@ def test_stencil_equals_the_staged_oracle():
     cc = [oms[0].cc.copy(), oms[1].cc.copy()]
     U = [np.column_stack([1 + 0.1 * np.sin(c[:, 0]), 1 + 0.1 * np.cos(c[:, 1]), 0.2 + 0.05 * c[:, 0],
                           0.01 * c[:, 1]]) for c in cc]
+    res = cp.receptor_states(U)
+    staged = np.concatenate([res[0][1], res[1][1]])
+    Uall = np.concatenate(U)
+    comp = np.stack([np.add.reduceat(st["w"] * Uall[st["donor"], j], st["off"][:-1]) for j in range(4)], 1)
+    assert np.max(np.abs(comp - staged)) <= 1e-14 * np.max(np.abs(staged))
+
+
+@functools.lru_cache(maxsize=None)
+def ref_pair():
+    """The reference's own overset scenario: part1.grd and part2.grd moved by
+    (-0.7, 0.15) (Orchestra.cpp:204), wall tag -2, overset tag -2147483647,
+    mexclude 1, nfringe 2 (test.input, read at Orchestra.cpp:105-110)."""
+    import os
+    g = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    p1 = mesh.read_grd(os.path.join(g, "part1.grd"), -2, -2147483647)
+    p2 = mesh.read_grd(os.path.join(g, "part2.grd"), -2, -2147483647)
+    p2.xy = p2.xy + np.array([-0.7, 0.15])
+    flags = [(m.frc == mesh.FARFIELD).astype(np.int32) for m in (p1, p2)]
+    conn, st = usg.overset_coupling([p1, p2], flags, mexclude=1, nfringe=2)
+    return p1, p2, conn, st
+
+
+def test_reference_pair_stencil_equals_the_staged_oracle():
+    """On the reference's part1 + part2 pair: the same blanking and receptor
+    cells as the oracle's staged coupling, the same receptor states for a
+    smooth field, donors that are field cells and weights that sum to one.
+    Both meshes receive (part1 around part2's element, part2 on its overset
+    boundary where part1 covers it)."""
+    p1, p2, conn, st = ref_pair()
+    oms = [O.OracleMesh(p1), O.OracleMesh(p2)]
+    cp = OversetCoupling(oms, conn)
+    assert np.array_equal(np.concatenate(cp.blank), st["blank"])
+    assert np.array_equal(np.concatenate([cp.recv[0], cp.recv[1] + p1.nc]), st["recv"])
+    n1 = int((st["recv"] < p1.nc).sum())
+    assert n1 > 0 and len(st["recv"]) - n1 > 0
+    assert (st["blank"][:p1.nc] == 0).sum() > 0                 # part1 is cut by part2's wall
+    assert np.all(st["blank"][st["donor"]] == 1)
+    assert np.allclose(np.add.reduceat(st["w"], st["off"][:-1]), 1.0, rtol=0, atol=4e-16)
+    cc = [oms[0].cc.copy(), oms[1].cc.copy()]
+    U = [np.column_stack([1 + 0.1 * np.sin(3 * c[:, 0]), 1 + 0.1 * np.cos(2 * c[:, 1]), 0.3 + 0.05 * c[:, 0],
+                          0.02 * c[:, 1]]) for c in cc]
     res = cp.receptor_states(U)
     staged = np.concatenate([res[0][1], res[1][1]])
     Uall = np.concatenate(U)

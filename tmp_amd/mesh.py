@@ -738,3 +738,61 @@ def cylinder_ogrid(ni: int = 256, nj: int = 96, d: float = 1.0, dr1: float = 1e-
     m = faces_from_cells(xy, cells, nv, bc)
     m.name = f"cylinder_ogrid_{ni}x{nj}"
     return m, (m.frc == FARFIELD).astype(np.int32)
+
+
+def cylinder_delaunay(n_cells: int = 1_000_000, d: float = 1.0, r_far: float = 30.0, alpha: float = 0.6,
+                      seed: int = 7) -> Mesh:
+    """Irregular triangulation around a cylinder of diameter d: points on
+    rings whose spacing grows as h(r) = h_w (2r/d)^alpha, each ring with its
+    own point count round(2 pi r / h) and a random angular phase, interior
+    points jittered by U(-0.2, 0.2) h radially and along the ring, then the
+    Delaunay triangulation (scipy) with the triangles inside the cylinder
+    removed.  Ring counts that change from ring to ring and the random
+    phases give node valences from 4 to 8, unlike the structured and
+    split-quad meshes.  h_w is scaled so that the mesh has about n_cells
+    triangles.  Wall on the cylinder, far field on the circle r = r_far."""
+    from scipy.spatial import Delaunay
+
+    rng = np.random.default_rng(seed)
+    r0 = 0.5 * d
+
+    def rings(hw):
+        rs = [r0]
+        while True:
+            h = hw * (rs[-1] / r0) ** alpha
+            r = rs[-1] + 0.866 * h
+            if r >= r_far:
+                break
+            rs.append(r)
+        rs.append(r_far)
+        rs = np.array(rs)
+        h = hw * (rs / r0) ** alpha
+        return rs, h, np.maximum(8, np.round(2 * np.pi * rs / h).astype(np.int64))
+
+    hw = 0.01
+    for _ in range(6):           # triangles ~ 2 x points
+        _, _, n = rings(hw)
+        hw *= np.sqrt(2.0 * n.sum() / n_cells)
+    rs, h, n = rings(hw)
+    pts = []
+    for k, (r, hk, nk) in enumerate(zip(rs, h, n)):
+        th = 2 * np.pi * (np.arange(nk) + rng.uniform()) / nk
+        rr = np.full(nk, r)
+        if 0 < k < len(rs) - 1:
+            rr = rr + rng.uniform(-0.2, 0.2, nk) * hk
+            th = th + rng.uniform(-0.2, 0.2, nk) * hk / r
+        pts.append(np.column_stack([rr * np.cos(th), rr * np.sin(th)]))
+    xy = np.concatenate(pts)
+    tri = Delaunay(xy).simplices.astype(np.int64)
+    c = xy[tri].mean(axis=1)
+    tri = tri[np.hypot(c[:, 0], c[:, 1]) > r0]
+    cells, nv = _assemble(xy, np.zeros((0, 4), np.int64), tri)
+    rmid = 0.5 * (r0 + r_far)
+
+    def bc(x0, y0, x1, y1):
+        return np.where(np.hypot(0.5 * (x0 + x1), 0.5 * (y0 + y1)) < rmid, WALL, FARFIELD)
+
+    m = faces_from_cells(xy, cells, nv, bc)
+    m.name = f"cylinder_delaunay_{m.nc}"
+    return m
+
