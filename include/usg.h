@@ -375,8 +375,10 @@ int usg_overset_cell_stencil(usg_overset *ov, const usg_mesh_desc *meshes, int n
  * iteration first sets each receptor cell's state from its donors,
  * U_r = sum_i w_i U_donor(i) and W_r = cons(U_r); the sweeps then update
  * field cells only (hole and receptor cells keep their state and take no
- * increment, so a field cell sees dW = 0 across the overset boundary), and
- * the residual norm sums field cells only.  Receptors must be distinct
+ * increment, so a field cell sees dW = 0 across the overset boundary), hole
+ * and receptor cells reconstruct first order (a receptor's gradient reaches
+ * into the hole cells beside it), and the residual norm sums field cells
+ * only.  Receptors must be distinct
  * cells blanked -1 and donors cells blanked 1.  blank NULL removes the
  * coupling.  The reference registers node arrays with tioga and calls
  * dataUpdate from its driver (Orchestra.cpp:233-292); this is that step
@@ -395,8 +397,8 @@ int usg_set_cfl(usg_ctx *ctx, double cfl);
  * the reference's limited linear reconstruction (HLLCFluxStrategy.h:35-100,
  * the default), 1 uses the cell values (limiter phi = 0).  No counterpart
  * in the reference.  On meshes with y+ ~ 1 wall spacing the second-order
- * scheme diverges at any CFL number (Euler, laminar and SST alike); the
- * first-order one converges. */
+ * scheme diverged at every CFL number tried (Euler, laminar and SST); the
+ * first-order one converges (SST on the 1M-cell airfoil: 4 decades). */
 int usg_set_order(usg_ctx *ctx, int order);
 
 /* Dual time stepping for unsteady flow (Euler and laminar; no counterpart

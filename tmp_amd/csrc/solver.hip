@@ -2413,11 +2413,7 @@ static void dfree(usg_ctx *c, void *p) {
   (void)hipFree(p);
 }
 
-int usg_set_overset(usg_ctx *c, const int *blank, int nrecv, const int *recv, const int *off, const int *donor,
-                    const double *w) {
-  if (!c) return fail(USG_ERR_ARG, "null ctx");
-  HIPCHK(hipStreamSynchronize(c->stream));
-  drop_graphs(c);
+static void clear_overset(usg_ctx *c) {
   dfree(c, c->d_ov_recv);
   dfree(c, c->d_ov_off);
   dfree(c, c->d_ov_donor);
@@ -2428,6 +2424,15 @@ int usg_set_overset(usg_ctx *c, const int *blank, int nrecv, const int *recv, co
   c->d_blank = nullptr;
   c->dm.blank = nullptr;
   c->ov_n = 0;
+}
+
+// Any failure leaves the context without a coupling (as blank NULL does).
+int usg_set_overset(usg_ctx *c, const int *blank, int nrecv, const int *recv, const int *off, const int *donor,
+                    const double *w) {
+  if (!c) return fail(USG_ERR_ARG, "null ctx");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  clear_overset(c);
   if (!blank) return USG_OK;
   if (c->prm.flow == USG_SST) return fail(USG_ERR_ARG, "overset coupling is provided for Euler and laminar flow");
   const int nc = c->hm.nc;
@@ -2452,14 +2457,19 @@ int usg_set_overset(usg_ctx *c, const int *blank, int nrecv, const int *recv, co
         return fail(USG_ERR_ARG, "receptor " + std::to_string(r) + " has a donor that is not a field cell");
   }
   int rc;
-  if ((rc = upload(c, &c->d_blank, b.data(), nc))) return rc;
-  c->dm.blank = c->d_blank;
+  if ((rc = upload(c, &c->d_blank, b.data(), nc))) {
+    clear_overset(c);
+    return rc;
+  }
   if (nrecv) {
     const int nd = off[nrecv];
     if ((rc = upload(c, &c->d_ov_recv, recv, nrecv)) || (rc = upload(c, &c->d_ov_off, off, nrecv + 1)) ||
-        (rc = upload(c, &c->d_ov_donor, donor, nd)) || (rc = upload(c, &c->d_ov_w, w, nd)))
+        (rc = upload(c, &c->d_ov_donor, donor, nd)) || (rc = upload(c, &c->d_ov_w, w, nd))) {
+      clear_overset(c);
       return rc;
+    }
   }
+  c->dm.blank = c->d_blank;
   c->ov_n = nrecv;
   HIPCHK(hipStreamSynchronize(c->stream));
   return USG_OK;
