@@ -18,12 +18,16 @@ constexpr int kXcd = 8;                    // XCDs (and spatial parts) of an MI3
 struct XSlice {
   int off[kXcd + 1];
 };
-__device__ __forceinline__ bool xcd_item(const int *__restrict__ items, const XSlice &sl, int &item) {
-  const int part = blockIdx.x % kXcd;
-  const int i = sl.off[part] + (int)(blockIdx.x / kXcd) * (int)blockDim.x + (int)threadIdx.x;
+// (block b of the list's own blocks: b % kXcd is the part, b / kXcd the chunk)
+__device__ __forceinline__ bool xcd_item_at(const int *__restrict__ items, const XSlice &sl, int b, int &item) {
+  const int part = b % kXcd;
+  const int i = sl.off[part] + (b / kXcd) * (int)blockDim.x + (int)threadIdx.x;
   if (i >= sl.off[part + 1]) return false;
   item = items[i];
   return true;
+}
+__device__ __forceinline__ bool xcd_item(const int *__restrict__ items, const XSlice &sl, int &item) {
+  return xcd_item_at(items, sl, (int)blockIdx.x, item);
 }
 constexpr int kWallD = -2;
 constexpr int kFarD = -3;

@@ -890,10 +890,181 @@ struct NormBlocks {
   int *ctr = nullptr;
 };
 
+// DF of one dependency neighbour nb of cell c across slot sl (SolveDF,
+// TopologyHolderStrategy.cpp:217-323; SST SolveDF, SSTKWStrategy.h:251-417),
+// given nb's increments (dW1 in the forward sweep, dW in the backward one):
+// the term t that the cell's sum adds.
+template <int FLOW>
+__device__ __forceinline__ void df_term(const DevMesh &m, const DevState &s, int c, int sl, bool left, int nb,
+                                        const double *dW, const double *tdw, double *t, double *tt) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  const size_t sc = (size_t)sl * nc + c;
+  const double n0 = m.snx[sc], n1 = m.sny[sc], A = m.sA[sc];
+  double W[4], Wp[4], Fc[4], Fcp[4];
+  for (int j = 0; j < 4; j++) {
+    const double W0 = s.W[j * nc + nb];
+    Wp[j] = W0 + dW[j];
+    W[j] = W0;
+  }
+  cons_flux(W, n0, n1, Fc);
+  cons_flux(Wp, n0, n1, Fcp);
+  const double rA = s.rA[sc];
+  for (int j = 0; j < 4; j++) {
+    double DFc = left ? (Fcp[j] - Fc[j]) : (Fc[j] - Fcp[j]);
+    double Fv = rA * dW[j];
+    t[j] = 0.5 * (DFc - Fv) * A;
+  }
+  if (F::sst) {
+    double u = W[2] / W[0], v = W[3] / W[0];
+    double vn = dot2(u, v, n0, n1);
+    double up = Wp[2] / Wp[0], vp = Wp[3] / Wp[0];
+    double vnp = dot2(up, vp, n0, n1);
+    for (int q = 0; q < 2; q++) {
+      const double tw = s.tW[q * nc + nb];
+      double twp = tw + tdw[q];
+      double fc = tw * vn, fcp = twp * vnp;
+      double DFc = left ? (fcp - fc) : (fc - fcp);
+      double Fv = rA * tdw[q];
+      tt[q] = 0.5 * (DFc - Fv) * A;
+    }
+  }
+}
+
+// Sum of DF over the dependency neighbours of cell c -- lower ones in the
+// forward sweep, upper ones in the backward sweep -- in slot order, from
+// their stored increments.
+template <int FLOW, bool FWD>
+__device__ __forceinline__ void df_sum(const DevMesh &m, const DevState &s, int c, double *dFi, double *tFi) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  const double *dsrc = FWD ? s.dW1 : s.dW;
+  const double *tsrc = FWD ? s.tdW1 : s.tdW;
+  for (int sl = 0; sl < 4; sl++) {
+    Slot o;
+    if (!get_slot(m, sl, c, o)) break;
+    if (o.nb < 0) continue;
+    if (FWD ? !(o.nb < c) : !(o.nb > c)) continue;
+    double dW[4], tdw[2] = {0.0, 0.0}, t[4], tt[2] = {0.0, 0.0};
+    for (int j = 0; j < 4; j++) dW[j] = dsrc[j * nc + o.nb];
+    if (F::sst)
+      for (int q = 0; q < 2; q++) tdw[q] = tsrc[q * nc + o.nb];
+    df_term<FLOW>(m, s, c, sl, o.left, o.nb, dW, tdw, t, tt);
+    for (int j = 0; j < 4; j++) dFi[j] += t[j];
+    if (F::sst)
+      for (int q = 0; q < 2; q++) tFi[q] += tt[q];
+  }
+}
+
+// The increment a sweep stores for cell u (fwd_store / bwd_store), from the
+// stored increments of its dependency neighbours: the same expressions, so
+// the same bits.
+template <int FLOW, bool FWD, bool B>
+__device__ __forceinline__ void increment_of(const DevMesh &m, const DevState &s, int u, double *dW, double *tdw) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  for (int j = 0; j < 4; j++) dW[j] = 0.0;
+  tdw[0] = tdw[1] = 0.0;
+  if (!FWD) {
+    if constexpr (B) {
+      if (s.err[2 + m.ccase[u]]) return;  // a failed batch case carries its state (dW = 0)
+    }
+  }
+  if (frozen_cell(m, u)) return;
+  double dFu[4] = {0, 0, 0, 0}, tFu[2] = {0, 0};
+  df_sum<FLOW, FWD>(m, s, u, dFu, tFu);
+  const double diag = s.D[u];
+  if (FWD) {
+    for (int j = 0; j < 4; j++) dW[j] = (-s.R[j * nc + u] - dFu[j]) / diag;
+    if (F::sst)
+      for (int q = 0; q < 2; q++) tdw[q] = (-s.tR[q * nc + u] - tFu[q]) / s.tD[q * nc + u];
+  } else {
+    for (int j = 0; j < 4; j++) dW[j] = s.dW1[j * nc + u] - dFu[j] / diag;
+    if (F::sst)
+      for (int q = 0; q < 2; q++) tdw[q] = s.tdW1[q * nc + u] - tFu[q] / s.tD[q * nc + u];
+  }
+}
+
+// The small last wave of a sweep (the 64 cells of the fourth colour on the
+// 1 M-cell airfoil) runs in the launch of the wave before it instead of a
+// launch of its own, which would cost a whole launch's latency for a
+// handful of cells.  Its blocks come first (a multiple of kXcd, so that the
+// wave's own blocks keep their part-to-XCD mapping) and start at once, so
+// their longer chains of loads overlap the wave's own work.  A tail cell's
+// dependency neighbour in the wave of the same launch (wflag bit 2 forward,
+// bit 3 backward) may not have stored its increment yet, so the tail
+// recomputes it (increment_of).
+struct SweepTail {
+  XSlice sl;
+  int nblk = 0;
+};
+
+// One tail cell with four lanes, lane q on face slot q: each lane finds its
+// dependency neighbour's increment (recomputed or loaded) and DF term; the
+// terms are then added in slot order, as df_sum adds them, and lane 0
+// stores.
 template <int FLOW, bool FWD, typename T, bool B>
-__global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevState s, const Consts k_,
-                                                 const int *__restrict__ items, const XSlice sl,
-                                                 const NormBlocks nb) {
+__device__ __forceinline__ void sweep_tail_cell(const DevMesh &m, const DevState &s, const Consts &k_,
+                                                const int *__restrict__ items, const SweepTail &tl) {
+  using F = Flow<FLOW>;
+  const int b = (int)blockIdx.x, part = b % kXcd;
+  const int i = tl.sl.off[part] + (b / kXcd) * ((int)blockDim.x >> 2) + ((int)threadIdx.x >> 2);
+  if (i >= tl.sl.off[part + 1]) return;  // the four lanes of a cell leave together
+  const int q = (int)threadIdx.x & 3;
+  const int c = items[i];
+  const int nc = m.nc;
+  double t[4] = {0.0, 0.0, 0.0, 0.0}, tt[2] = {0.0, 0.0};
+  int has = 0;
+  Slot o;
+  if (get_slot(m, q, c, o) && o.nb >= 0 && (FWD ? o.nb < c : o.nb > c)) {
+    has = 1;
+    double dW[4], tdw[2] = {0.0, 0.0};
+    if (m.wflag[o.nb] & (FWD ? 4 : 8)) {
+      increment_of<FLOW, FWD, B>(m, s, o.nb, dW, tdw);
+    } else {
+      const double *dsrc = FWD ? s.dW1 : s.dW;
+      const double *tsrc = FWD ? s.tdW1 : s.tdW;
+      for (int j = 0; j < 4; j++) dW[j] = dsrc[j * nc + o.nb];
+      if (F::sst)
+        for (int r = 0; r < 2; r++) tdw[r] = tsrc[r * nc + o.nb];
+    }
+    df_term<FLOW>(m, s, c, q, o.left, o.nb, dW, tdw, t, tt);
+  }
+  double dFi[4] = {0, 0, 0, 0}, tFi[2] = {0, 0};
+  for (int sl = 0; sl < 4; sl++) {
+    const int h = __shfl(has, sl, 4);
+    double tk[4], ttk[2];
+    for (int j = 0; j < 4; j++) tk[j] = __shfl(t[j], sl, 4);
+    for (int r = 0; r < 2; r++) ttk[r] = __shfl(tt[r], sl, 4);
+    if (h) {
+      for (int j = 0; j < 4; j++) dFi[j] += tk[j];
+      if (F::sst)
+        for (int r = 0; r < 2; r++) tFi[r] += ttk[r];
+    }
+  }
+  if (q != 0) return;
+  const Consts k = case_consts<B>(m, k_, c);
+  const double diag = s.D[c];
+  const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+  if (FWD) {
+    double dW1[4], tdW1[2];
+    fwd_store<FLOW>(m, s, c, dFi, tFi, diag, dW1, tdW1);
+    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
+  } else {
+    double dW1[4], tdW1[2] = {0.0, 0.0};
+    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
+    if (F::sst)
+      for (int r = 0; r < 2; r++) tdW1[r] = s.tdW1[r * nc + c];
+    bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, dFi, tFi, diag);
+  }
+}
+
+// Held to 96 VGPRs (5 waves per SIMD, as before the tail path existed; the
+// inlined recomputation of the tail path would otherwise take 106).
+template <int FLOW, bool FWD, typename T, bool B>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) void k_sweep(
+    const DevMesh m, const DevState s, const Consts k_, const int *__restrict__ items, const XSlice sl,
+    const NormBlocks nb, const SweepTail tl) {
   using F = Flow<FLOW>;
   if (nb.kind && (int)blockIdx.x >= nb.nsweep) {
     if (nb.kind == 1)
@@ -902,57 +1073,18 @@ __global__ __launch_bounds__(kBlock) void k_sweep(const DevMesh m, const DevStat
       norm_ring_block(nb.partial, nb.boff, nb.ring, nb.cap, nb.ctr, 0, 1);
     return;
   }
+  if ((int)blockIdx.x < tl.nblk) {
+    sweep_tail_cell<FLOW, FWD, T, B>(m, s, k_, items, tl);
+    return;
+  }
   int c;
-  if (!xcd_item(items, sl, c)) return;
+  if (!xcd_item_at(items, sl, (int)blockIdx.x - tl.nblk, c)) return;
   const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc;
   const double diag = s.D[c];
-  const double *dsrc = FWD ? s.dW1 : s.dW;
-  const double *tsrc = FWD ? s.tdW1 : s.tdW;
   double dFi[4] = {0, 0, 0, 0};
   double tFi[2] = {0, 0};
-  for (int sl = 0; sl < 4; sl++) {
-    Slot o;
-    if (!get_slot(m, sl, c, o)) break;
-    if (o.nb < 0) continue;
-    if (FWD ? !(o.nb < c) : !(o.nb > c)) continue;
-    const int nb = o.nb;
-    const size_t sc = (size_t)sl * nc + c;
-    const double n0 = m.snx[sc], n1 = m.sny[sc], A = m.sA[sc];
-    double W[4], Wp[4], dW[4], Fc[4], Fcp[4];
-    for (int j = 0; j < 4; j++) {
-      double W0 = s.W[j * nc + nb];
-      dW[j] = dsrc[j * nc + nb];
-      Wp[j] = W0 + dW[j];
-      W[j] = W0;
-    }
-    cons_flux(W, n0, n1, Fc);
-    cons_flux(Wp, n0, n1, Fcp);
-    const double rA = s.rA[sc];
-    for (int j = 0; j < 4; j++) {
-      double DFc = o.left ? (Fcp[j] - Fc[j]) : (Fc[j] - Fcp[j]);
-      double Fv = rA * dW[j];
-      dFi[j] += 0.5 * (DFc - Fv) * A;
-    }
-    if (F::sst) {
-      double tw[2], tdw[2];
-      for (int q = 0; q < 2; q++) {
-        tw[q] = s.tW[q * nc + nb];
-        tdw[q] = tsrc[q * nc + nb];
-      }
-      double u = W[2] / W[0], v = W[3] / W[0];
-      double vn = dot2(u, v, n0, n1);
-      double up = Wp[2] / Wp[0], vp = Wp[3] / Wp[0];
-      double vnp = dot2(up, vp, n0, n1);
-      for (int q = 0; q < 2; q++) {
-        double twp = tw[q] + tdw[q];
-        double fc = tw[q] * vn, fcp = twp * vnp;
-        double DFc = o.left ? (fcp - fc) : (fc - fcp);
-        double Fv = rA * tdw[q];
-        tFi[q] += 0.5 * (DFc - Fv) * A;
-      }
-    }
-  }
+  df_sum<FLOW, FWD>(m, s, c, dFi, tFi);
   const double zero[4] = {0.0, 0.0, 0.0, 0.0};
   if (FWD) {
     double dW1[4], tdW1[2];
@@ -1382,6 +1514,9 @@ struct usg_ctx {
   // LDS row length (the most faces of a tile)
   double *d_uedge = nullptr;  // face states for the U_edge getter, [4][nf]
   bool fused = false;
+  // the small last wave of the forward / backward sweep runs as a tail of
+  // the wave before it (SweepTail)
+  bool fwd_tail = false, bwd_tail = false;
   int tile_bs = 0, tile_stride = 0;
   int *d_tfoff = nullptr, *d_tface = nullptr;
   unsigned short *d_lfs = nullptr;
@@ -1591,21 +1726,45 @@ static void launch_residual_any(usg_ctx *c) {
   launch_residual_events(c, e.first, e.second);
 }
 
+// A sweep's last wave runs as a tail of the wave before it when it holds
+// at most kTailMax cells and at least min_waves waves exist (the forward
+// sweep keeps two launches of its own for the norm blocks: 4; backward: 3).
+constexpr int kTailMax = 8192;
+static bool sweep_tail(const std::vector<int> &poff, int min_waves) {
+  const int nw = (int)poff.size() / (kXcd + 1);
+  if (nw < min_waves) return false;
+  const int *p = &poff[(size_t)(nw - 1) * (kXcd + 1)];
+  return p[kXcd] - p[0] <= kTailMax;
+}
+
+// The last wave's items as a tail of the wave before it (SweepTail).
+static SweepTail tail_of(const std::vector<int> &poff, int bs) {
+  const int nw = (int)poff.size() / (kXcd + 1);
+  SweepTail tl;
+  tl.sl = slice(&poff[(size_t)(nw - 1) * (kXcd + 1)]);
+  tl.nblk = xgrid(tl.sl, bs / 4);  // four lanes per cell
+  return tl;
+}
+
 // With fuse_norm, the first two forward launches also carry the residual
 // norm of this iteration into the ring (NormBlocks).
 template <int FLOW, typename T, bool B>
 static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
   const HostMesh &h = c->hm;
   const int bs = c->blk[USG_K_SWEEP];
-  // wave 0 of each sweep is fused into k_cell / the forward waves (wflag)
+  // wave 0 of each sweep is fused into k_cell / the forward waves (wflag);
+  // a small last wave runs as a tail of the wave before it
   const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
-  for (int w = 1; w < nw; w++) {
+  for (int w = 1; w < nw - (c->fwd_tail ? 1 : 0); w++) {
     XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
+    const int nmain = xgrid(sl, bs);
+    const SweepTail tl = c->fwd_tail && w == nw - 2 ? tail_of(h.fwd_poff, bs) : SweepTail{};
+    const int ntail = tl.nblk;
     NormBlocks nb;
     int extra = 0;
     if (fuse_norm && w <= 2) {
       nb.kind = w;
-      nb.nsweep = xgrid(sl, bs);
+      nb.nsweep = nmain + ntail;
       nb.cap = kRing;
       nb.R = c->ds.R;
       nb.b0 = c->d_nb0;
@@ -1616,14 +1775,17 @@ static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
       nb.ctr = c->d_ctr;
       extra = w == 1 ? c->nnb : 1;
     }
-    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(xgrid(sl, bs) + extra), dim3(bs), 0, c->stream, c->dm,
-                       c->ds, c->k, c->d_fwd, sl, nb);
+    hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(nmain + ntail + extra), dim3(bs), 0, c->stream, c->dm,
+                       c->ds, c->k, c->d_fwd, sl, nb, tl);
   }
   const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
-  for (int w = 1; w < nb; w++) {
+  for (int w = 1; w < nb - (c->bwd_tail ? 1 : 0); w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
-    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(xgrid(sl, bs)), dim3(bs), 0, c->stream, c->dm, c->ds,
-                       c->k, c->d_bwd, sl, NormBlocks{});
+    const int nmain = xgrid(sl, bs);
+    const SweepTail tl = c->bwd_tail && w == nb - 2 ? tail_of(h.bwd_poff, bs) : SweepTail{};
+    const int ntail = tl.nblk;
+    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(nmain + ntail), dim3(bs), 0, c->stream, c->dm, c->ds,
+                       c->k, c->d_bwd, sl, NormBlocks{}, tl);
   }
 }
 
@@ -1849,12 +2011,24 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, (double **)&dm.fcx, fcx.data(), nf));
   CK(upload(c, (double **)&dm.fcy, fcy.data(), nf));
   {
-    // bit 0: forward wave 0 (no lower neighbour), bit 1: backward wave 0 (no upper neighbour)
+    // bit 0: forward wave 0 (no lower neighbour), bit 1: backward wave 0 (no
+    // upper neighbour); bit 2 (3): the cell lies in the last-but-one forward
+    // (backward) wave and the small last wave runs in that wave's launch
     std::vector<unsigned char> wflag(nc, 0);
     if (!h.fwd_poff.empty())
       for (int i = h.fwd_poff[0]; i < h.fwd_poff[kXcd]; i++) wflag[h.fwd_list[i]] |= 1;
     if (!h.bwd_poff.empty())
       for (int i = h.bwd_poff[0]; i < h.bwd_poff[kXcd]; i++) wflag[h.bwd_list[i]] |= 2;
+    c->fwd_tail = sweep_tail(h.fwd_poff, 4);
+    c->bwd_tail = sweep_tail(h.bwd_poff, 3);
+    for (int d = 0; d < 2; d++) {
+      const std::vector<int> &poff = d ? h.bwd_poff : h.fwd_poff;
+      const std::vector<int> &list = d ? h.bwd_list : h.fwd_list;
+      if (!(d ? c->bwd_tail : c->fwd_tail)) continue;
+      const int w = (int)poff.size() / (kXcd + 1) - 2;
+      for (int i = poff[(size_t)w * (kXcd + 1)]; i < poff[(size_t)w * (kXcd + 1) + kXcd]; i++)
+        wflag[list[i]] |= d ? 8 : 4;
+    }
     CK(upload(c, (unsigned char **)&dm.wflag, wflag.data(), wflag.size()));
   }
   CK(upload(c, &c->d_fwd, h.fwd_list.data(), nc));
