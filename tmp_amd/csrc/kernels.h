@@ -68,6 +68,10 @@ struct DevMesh {
   // 0 hole, -1 receptor.  Only field cells are updated by the sweeps and
   // enter the residual norm; receptors are set from their donors.
   const signed char *blank;
+  // boundary faces: bface[nbf] (ascending), and each face's index in that
+  // list (bidx[f], -1 for an interior face)
+  const int *bface, *bidx;
+  int nbf;
 };
 
 __device__ __forceinline__ bool frozen_cell(const DevMesh &m, int c) { return m.blank && m.blank[c] != 1; }
@@ -101,6 +105,9 @@ struct DevState {
   // dual time stepping: the conservative state at the two previous
   // physical time levels, W^n and W^(n-1) ([4][nc]; null when off)
   double *Wt1, *Wt2;
+  // face states of the boundary faces for the gradient kernel, [4][nbf]
+  // (k_bstate at the start of every residual)
+  double *Ub;
   // FP32 storage of the mixed-precision mode (null in FP64 mode, where the
   // double arrays above hold these quantities)
   float *gradf, *phif, *gradTf, *tgradf, *F1f, *F2f, *FFf;

@@ -81,6 +81,29 @@ __device__ __forceinline__ bool get_slot(const DevMesh &m, int s, int c, Slot &o
 }
 
 // ---------------------------------------------------------------------------
+// k_bstate: the face state of every boundary face (BoundaryStrategy::
+// HandleInvicidBoundary / HandleViscousBoundary, BoundaryStrategy.h:73-335)
+// from its cell's current state, for the gradient kernel, which would
+// otherwise hold the far-field Riemann state's registers (50 VGPRs, one
+// wave per SIMD) for the few cells that have a boundary face.  The slip-wall
+// check is counted here.
+// ---------------------------------------------------------------------------
+template <bool VISC, bool B>
+__global__ __launch_bounds__(kBlock) void k_bstate(const DevMesh m, const DevState s, const Consts k_) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= m.nbf) return;
+  const int f = m.bface[b], c = m.flc[f], nc = m.nc;
+  const Consts k = case_consts<B>(m, k_, c);
+  double Ul[4], Ue[4];
+  load4(s.U, nc, c, Ul);
+  if (!boundary_state<VISC>(k, m.frc[f], Ul, m.fnx[f], m.fny[f], Ue)) {
+    atomicAdd(&s.err[1], 1);
+    if constexpr (B) atomicOr(&s.err[2 + m.ccase[c]], 2);
+  }
+  for (int j = 0; j < 4; j++) s.Ub[(size_t)j * m.nbf + b] = Ue[j];
+}
+
+// ---------------------------------------------------------------------------
 // k_grad: boundary states + gradients + limiter (+ SST F1/F2)
 // Euler2D::updateEdgeValues/solveGradient (Euler2D.cpp:202-240),
 // ViscousFlow2D.cpp:174-252, Vankatakrishnan_Limiter.hpp:69-200,
@@ -149,7 +172,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   double umaxd[4] = {Ucd[0], Ucd[1], Ucd[2], Ucd[3]};
   double umind[4] = {Ucd[0], Ucd[1], Ucd[2], Ucd[3]};
   const double Tcd = Ucd[1] / Ucd[0];
-  int bad = 0;
   for (int sl = 0; sl < 4; sl++) {
     Slot o;
     if (!get_slot(m, sl, c, o)) break;
@@ -173,7 +195,9 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
         }
       }
     } else {
-      if (!boundary_state<F::visc>(k, o.nb, Ucd, m.fnx[o.f], m.fny[o.f], Ued)) bad = 1;
+      // the face state k_bstate formed for this residual
+      const int b = m.bidx[o.f];
+      for (int j = 0; j < 4; j++) Ued[j] = s.Ub[(size_t)j * m.nbf + b];
       for (int j = 0; j < 4; j++) Ue[j] = kDiff ? T(Ued[j] - Ucd[j]) : T(Ued[j]);
     }
     // Green-Gauss: + for the left cell, - for the right cell
@@ -262,11 +286,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
     muc = sutherland<double>(k, Tc);
     s.mu[c] = muc;
   }
-  if (bad) {
-    atomicAdd(&s.err[1], 1);
-    if constexpr (B) atomicOr(&s.err[2 + m.ccase[c]], 2);
-  }
-
   // Venkatakrishnan limiter: eps^2 = 5^3 * V_left (integer 3/ndim == 1)
   T d1max[4], d1min[4], ph[4] = {1.0, 1.0, 1.0, 1.0};
   for (int j = 0; j < 4; j++) {
@@ -1666,6 +1685,9 @@ template <int FLOW, typename T, bool B>
 static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   const XSlice cs = slice(c->hm.cell_off.data()), fs = slice(c->hm.face_off.data());
   const int bg = c->blk[USG_K_GRAD], bf = c->blk[USG_K_FACE], bc = c->blk[USG_K_CELL];
+  if (c->dm.nbf > 0)
+    hipLaunchKernelGGL((k_bstate<Flow<FLOW>::visc, B>), dim3(grid(c->dm.nbf)), dim3(kBlock), 0, c->stream, c->dm,
+                       c->ds, c->k);
   hipLaunchKernelGGL((k_grad<FLOW, T, B>), dim3(xgrid(cs, bg)), dim3(bg), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
   if constexpr (std::is_same<T, double>::value) {
@@ -2035,6 +2057,19 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, &c->d_bwd, h.bwd_list.data(), nc));
   CK(upload(c, &c->d_cell_list, h.cell_list.data(), nc));
   CK(upload(c, &c->d_face_list, h.face_list.data(), nf));
+  {
+    // boundary faces for k_bstate, and each face's index among them
+    std::vector<int> bface, bidx(nf, -1);
+    for (int f = 0; f < nf; f++)
+      if (h.frc[f] < 0) {
+        bidx[f] = (int)bface.size();
+        bface.push_back(f);
+      }
+    dm.nbf = (int)bface.size();
+    CK(upload(c, (int **)&dm.bface, bface.data(), std::max<size_t>(bface.size(), 1)));
+    CK(upload(c, (int **)&dm.bidx, bidx.data(), nf));
+    CK(dalloc(c, &c->ds.Ub, 4 * std::max<size_t>(bface.size(), 1)));
+  }
   std::vector<int> wc(h.wall_faces.size());
   for (size_t i = 0; i < wc.size(); i++) wc[i] = h.flc[h.wall_faces[i]];
   CK(upload(c, &c->d_wall_faces, h.wall_faces.data(), h.wall_faces.size()));
