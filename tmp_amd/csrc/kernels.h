@@ -42,21 +42,28 @@ struct DevMesh {
   const double *vol, *ccx, *ccy, *wdist;
   const int *cf, *cnb;
   const int *flc, *frc;  // face left cell, right cell or kWallD / kFarD
-  const double *fnx, *fny, *fA, *fcx, *fcy;
+  const double *fnx, *fny, *fA;
+  // face offsets fdx[6][nf] = (x_f - x_L, y_f - y_L, x_f - x_R, y_f - y_R,
+  // x_R - x_L, y_R - y_L) (the last four 0 on a boundary face), differenced
+  // on the host with the same FP64 subtractions the kernels would make, so a
+  // face reads them with its own record instead of gathering cell centres
+  const double *fdx;
   // |x_R - x_L| of interior faces (viscous flows; the r_A distance of
   // SolveViscousFlux, TopologyHolderStrategy.cpp:143-188)
   const double *fdist;
   // the face normal and length again, per cell slot ([s*nc + c], the slot
-  // order of cf), so that a sweep wave (a contiguous cell range) reads them
-  // coalesced; fslot[f] / fslot[nf + f]: the slot of face f in its left /
-  // right cell's list
+  // order of cf), so that a sweep wave (a contiguous cell range) and the
+  // gradient kernel read them coalesced; fslot[f] / fslot[nf + f]: the slot
+  // of face f in its left / right cell's list
   const double *snx, *sny, *sA;
   const int *fslot;
+  // x_f - x_c, y_f - y_c of slot s's face from cell c's centre ([s*nc + c],
+  // FP64, the limiter's offsets in the gradient kernel)
+  const double *sdx, *sdy;
   // FP32 copies for the mixed-precision kernels (null in FP64 mode):
-  // normal, length, and the offsets fdx[6][nf] = (x_f - x_L, y_f - y_L,
-  // x_f - x_R, y_f - y_R, x_R - x_L, y_R - y_L), differenced in FP64 on the
-  // host so that FP32 keeps their relative precision near the wall
-  const float *fnxf, *fnyf, *fAf, *fdx;
+  // normal, length and the offsets fdx, each rounded once from FP64 so that
+  // FP32 keeps their relative precision near the wall
+  const float *fnxf, *fnyf, *fAf, *fdxf;
   // batch of cases: constants of case i at kc[i], case of cell c at
   // ccase[c]; both null for a single case (the kernel argument holds them)
   const struct Consts *kc;

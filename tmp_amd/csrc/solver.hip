@@ -117,18 +117,14 @@ __device__ __forceinline__ const T *fny_of(const DevMesh &m) { return pickc<T>(m
 template <typename T>
 __device__ __forceinline__ const T *fA_of(const DevMesh &m) { return pickc<T>(m.fA, m.fAf); }
 
-// Offset x_f - x_c of face f from the centre of `cell` (its left cell when
-// `left`): differenced here in FP64, read precomputed in FP32.
+// Offset x_f - x_c of face f from the centre of its left (`left`) or right
+// cell, precomputed per face (FP64 differences, rounded once for FP32).
 template <typename T>
-__device__ __forceinline__ void face_offset(const DevMesh &m, int cell, int f, bool left, T &v0, T &v1) {
-  if constexpr (std::is_same<T, double>::value) {
-    v0 = m.fcx[f] - m.ccx[cell];
-    v1 = m.fcy[f] - m.ccy[cell];
-  } else {
-    const int nf = m.nf;
-    v0 = left ? m.fdx[f] : m.fdx[2 * nf + f];
-    v1 = left ? m.fdx[nf + f] : m.fdx[3 * nf + f];
-  }
+__device__ __forceinline__ void face_offset(const DevMesh &m, int f, bool left, T &v0, T &v1) {
+  const T *d = pickc<T>(m.fdx, m.fdxf);
+  const int nf = m.nf;
+  v0 = left ? d[f] : d[2 * nf + f];
+  v1 = left ? d[nf + f] : d[3 * nf + f];
 }
 
 // ---------------------------------------------------------------------------
@@ -175,7 +171,18 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   for (int sl = 0; sl < 4; sl++) {
     Slot o;
     if (!get_slot(m, sl, c, o)) break;
-    const T n0 = fnx[o.f], n1 = fny[o.f], A = fA[o.f];
+    // FP64: the slot's own copy of the normal and length (coalesced)
+    T n0, n1, A;
+    if constexpr (std::is_same<T, double>::value) {
+      const size_t sc = (size_t)sl * nc + c;
+      n0 = m.snx[sc];
+      n1 = m.sny[sc];
+      A = m.sA[sc];
+    } else {
+      n0 = fnx[o.f];
+      n1 = fny[o.f];
+      A = fA[o.f];
+    }
     T Ue[4];
     double Ued[4];
     if (o.nb >= 0) {
@@ -301,7 +308,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
     Slot o;
     if (!get_slot(m, sl, c, o)) break;
     T v0, v1;
-    face_offset<T>(m, c, o.f, o.left, v0, v1);
+    if constexpr (std::is_same<T, double>::value) {
+      v0 = m.sdx[(size_t)sl * nc + c];
+      v1 = m.sdy[(size_t)sl * nc + c];
+    } else {
+      face_offset<T>(m, o.f, o.left, v0, v1);
+    }
     T Vl = o.left ? Vt : T(m.vol[o.nb]);
     T eps2 = T(125.0) * Vl;
     for (int j = 0; j < 4; j++) {
@@ -343,7 +355,7 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
   const T *G = pickc<T>(s.grad, s.gradf);
   const T *P = pickc<T>(s.phi, s.phif);
   T v0, v1;
-  face_offset<T>(m, cell, f, left, v0, v1);
+  face_offset<T>(m, f, left, v0, v1);
   for (int j = 0; j < 4; j++) {
     T d2 = dot2(G[(2 * j) * nc + cell], G[(2 * j + 1) * nc + cell], v0, v1);
     out[j] = d2 * P[j * nc + cell] + Ucell[j];
@@ -465,13 +477,8 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   T gE[3][2];
   T l2r0 = 0, l2r1 = 0, l2sq = 0;
   if (inner) {
-    if constexpr (std::is_same<T, double>::value) {
-      l2r0 = m.ccx[Rr] - m.ccx[L];
-      l2r1 = m.ccy[Rr] - m.ccy[L];
-    } else {
-      l2r0 = m.fdx[4 * nf + f];
-      l2r1 = m.fdx[5 * nf + f];
-    }
+    l2r0 = pickc<T>(m.fdx, m.fdxf)[4 * nf + f];
+    l2r1 = pickc<T>(m.fdx, m.fdxf)[5 * nf + f];
     l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
     for (int j = 0; j < 2; j++) {
       T a0 = (G[(4 + 2 * j) * nc + L] + G[(4 + 2 * j) * nc + Rr]) * T(0.5);
@@ -2001,6 +2008,7 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   }
   std::vector<int> cf(4 * (size_t)nc, kNoSlot), cnb(4 * (size_t)nc, kNoSlot), fslot(2 * (size_t)nf, 0);
   std::vector<double> snx(4 * (size_t)nc, 0.0), sny(4 * (size_t)nc, 0.0), sA(4 * (size_t)nc, 0.0);
+  std::vector<double> sdx(4 * (size_t)nc, 0.0), sdy(4 * (size_t)nc, 0.0);
   for (int i = 0; i < nc; i++)
     for (int s = 0; s < h.cnf[i]; s++) {
       int f = h.cface[4 * i + s];
@@ -2011,7 +2019,21 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
       snx[(size_t)s * nc + i] = h.fn[2 * f];
       sny[(size_t)s * nc + i] = h.fn[2 * f + 1];
       sA[(size_t)s * nc + i] = h.fa[f];
+      sdx[(size_t)s * nc + i] = fcx[f] - ccx[i];
+      sdy[(size_t)s * nc + i] = fcy[f] - ccy[i];
     }
+  std::vector<double> fd(6 * (size_t)nf, 0.0);
+  for (int f = 0; f < nf; f++) {
+    const int L = h.flc[f], R = h.frc[f];
+    fd[f] = fcx[f] - ccx[L];
+    fd[nf + f] = fcy[f] - ccy[L];
+    if (R >= 0) {
+      fd[2 * (size_t)nf + f] = fcx[f] - ccx[R];
+      fd[3 * (size_t)nf + f] = fcy[f] - ccy[R];
+      fd[4 * (size_t)nf + f] = ccx[R] - ccx[L];
+      fd[5 * (size_t)nf + f] = ccy[R] - ccy[L];
+    }
+  }
   DevMesh &dm = c->dm;
   dm.nc = nc;
   dm.nf = nf;
@@ -2025,13 +2047,14 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   CK(upload(c, (double **)&dm.snx, snx.data(), snx.size()));
   CK(upload(c, (double **)&dm.sny, sny.data(), sny.size()));
   CK(upload(c, (double **)&dm.sA, sA.data(), sA.size()));
+  CK(upload(c, (double **)&dm.sdx, sdx.data(), sdx.size()));
+  CK(upload(c, (double **)&dm.sdy, sdy.data(), sdy.size()));
   CK(upload(c, (int **)&dm.flc, h.flc.data(), nf));
   CK(upload(c, (int **)&dm.frc, h.frc.data(), nf));
   CK(upload(c, (double **)&dm.fnx, fnx.data(), nf));
   CK(upload(c, (double **)&dm.fny, fny.data(), nf));
   CK(upload(c, (double **)&dm.fA, h.fa.data(), nf));
-  CK(upload(c, (double **)&dm.fcx, fcx.data(), nf));
-  CK(upload(c, (double **)&dm.fcy, fcy.data(), nf));
+  CK(upload(c, (double **)&dm.fdx, fd.data(), fd.size()));
   {
     // bit 0: forward wave 0 (no lower neighbour), bit 1: backward wave 0 (no
     // upper neighbour); bit 2 (3): the cell lies in the last-but-one forward
@@ -2078,26 +2101,18 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   const bool visc = params->flow != USG_EULER, sst = params->flow == USG_SST;
   const bool mixed = params->precision == USG_MIXED;
   if (mixed) {
-    // FP32 face geometry; the offsets are differenced in FP64 first
-    std::vector<float> nxf(nf), nyf(nf), af(nf), fd(6 * (size_t)nf, 0.0f);
+    // FP32 face geometry; the offsets are the FP64 differences, rounded once
+    std::vector<float> nxf(nf), nyf(nf), af(nf), fdf(fd.size());
     for (int f = 0; f < nf; f++) {
       nxf[f] = (float)fnx[f];
       nyf[f] = (float)fny[f];
       af[f] = (float)h.fa[f];
-      const int L = h.flc[f], R = h.frc[f];
-      fd[f] = (float)(fcx[f] - ccx[L]);
-      fd[nf + f] = (float)(fcy[f] - ccy[L]);
-      if (R >= 0) {
-        fd[2 * (size_t)nf + f] = (float)(fcx[f] - ccx[R]);
-        fd[3 * (size_t)nf + f] = (float)(fcy[f] - ccy[R]);
-        fd[4 * (size_t)nf + f] = (float)(ccx[R] - ccx[L]);
-        fd[5 * (size_t)nf + f] = (float)(ccy[R] - ccy[L]);
-      }
     }
+    for (size_t i = 0; i < fd.size(); i++) fdf[i] = (float)fd[i];
     CK(upload(c, (float **)&dm.fnxf, nxf.data(), nf));
     CK(upload(c, (float **)&dm.fnyf, nyf.data(), nf));
     CK(upload(c, (float **)&dm.fAf, af.data(), nf));
-    CK(upload(c, (float **)&dm.fdx, fd.data(), fd.size()));
+    CK(upload(c, (float **)&dm.fdxf, fdf.data(), fdf.size()));
   }
   CK(dalloc(c, &s.U, 4 * (size_t)nc));
   CK(dalloc(c, &s.W, 4 * (size_t)nc));

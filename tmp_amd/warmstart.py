@@ -20,6 +20,10 @@ sin(alpha), (p - 1) / (gamma M^2).
 Run ``python -m tmp_amd.warmstart`` for the cold-versus-warm experiment
 (prints progress and one JSON summary line); ``--checkpoints DIR`` keeps
 each batch of training solves as a checkpoint and reuses it on the next run.
+``--save-net FILE`` writes the trained network (weights, biases and the
+training figures, .npz) and ``--load-net FILE`` starts from such a file
+instead of building the dataset and training, so the two halves of a large
+run can go in separate processes.
 """
 from __future__ import annotations
 
@@ -258,14 +262,41 @@ def first_hits(hist: np.ndarray, tol: np.ndarray) -> np.ndarray:
 LEVELS = (1e-4, 1e-6, 1e-8)
 
 
+def save_net(path, net, n_train, n_kept, n_points, epochs, losses, t_train):
+    """The trained network and its training figures as one .npz."""
+    w, b = net.get_params()
+    np.savez(path, w=w, b=b, losses=np.asarray(losses, np.float64),
+             meta=np.array([n_train, n_kept, n_points, epochs], np.int64), t_train=np.float64(t_train))
+
+
+def load_net(path):
+    """(net, losses, t_train, (n_train, n_kept, n_points, epochs)) from save_net's file."""
+    with np.load(path, allow_pickle=False) as z:
+        net = _mlp.MLP(NIN, 256, 8, NOUT)
+        net.set_params(z["w"], z["b"])
+        return net, z["losses"].tolist(), float(z["t_train"]), tuple(int(v) for v in z["meta"])
+
+
 def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=40000, keep_drop=1e-5,
-               checkpoint_dir=None):
+               checkpoint_dir=None, save_path=None, load_path=None, train_only=False):
     t_all = time.perf_counter()
-    train_cases = family_cases(n_train, 5005)
     test_cases = family_cases(n_test, 5007)
-    X, Y, kept = make_dataset(train_cases, train_iters, keep_drop, checkpoint_dir=checkpoint_dir)
-    log(f"dataset: {len(kept)} fields, {len(X)} points")
-    net, losses, t_train = train_net(X, Y, epochs)
+    if load_path:
+        net, losses, t_train, (n_train, n_kept, n_points, epochs) = load_net(load_path)
+        log(f"network from {load_path}: {n_kept} of {n_train} training fields, {epochs} epochs")
+    else:
+        train_cases = family_cases(n_train, 5005)
+        X, Y, kept = make_dataset(train_cases, train_iters, keep_drop, checkpoint_dir=checkpoint_dir)
+        log(f"dataset: {len(kept)} fields, {len(X)} points")
+        net, losses, t_train = train_net(X, Y, epochs)
+        n_kept, n_points = len(kept), int(len(X))
+        del X, Y
+        if save_path:
+            save_net(save_path, net, n_train, n_kept, n_points, epochs, losses, t_train)
+            log(f"network saved to {save_path}")
+    if train_only:
+        return {"train_cases": n_kept, "train_points": n_points, "epochs": epochs, "train_seconds": t_train,
+                "final_loss": losses[-1], "saved_to": save_path, "total_seconds": time.perf_counter() - t_all}
     ms = meshes_for(test_cases)
     # cold: free stream; the tolerances are relative to its first residual
     cold = batch_solver(ms, test_cases)
@@ -312,7 +343,7 @@ def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=
     both = (n_cold > 0) & (n_warm > 0)
     res = {
         "family": "laminar NACA 4-digit, M 0.2-0.5, alpha -2..6, Re 1e3-3.2e3, 96-point hybrid mesh",
-        "train_cases": len(kept), "train_points": int(len(X)), "epochs": epochs, "train_seconds": t_train,
+        "train_cases": n_kept, "train_points": n_points, "epochs": epochs, "train_seconds": t_train,
         "final_loss": losses[-1], "test_cases": n_test, "max_iters": max_iters,
         "tolerances": "x the cold run's first residual", "levels": levels,
         "initial_residual_warm_over_cold": float(np.median(h_warm[0] / h_cold[0])),
@@ -348,6 +379,7 @@ def plain_solves(ms, cases, tol, max_iters, chunk=500):
                 break
         total += time.perf_counter() - t0
         s.close()
+        log(f"  plain solve {i + 1}/{len(cases)}: {done} iterations")
     return total
 
 
@@ -359,8 +391,12 @@ def main():
     ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--max-iters", type=int, default=40000)
     ap.add_argument("--checkpoints", default=None, help="directory of training-solve checkpoints (reused)")
+    ap.add_argument("--save-net", default=None, help="write the trained network to this .npz")
+    ap.add_argument("--load-net", default=None, help="start from this trained network (no dataset, no training)")
+    ap.add_argument("--train-only", action="store_true", help="stop after training (with --save-net)")
     a = ap.parse_args()
-    r = experiment(a.train, a.test, a.train_iters, a.epochs, a.max_iters, checkpoint_dir=a.checkpoints)
+    r = experiment(a.train, a.test, a.train_iters, a.epochs, a.max_iters, checkpoint_dir=a.checkpoints,
+                   save_path=a.save_net, load_path=a.load_net, train_only=a.train_only)
     print(json.dumps(r), flush=True)
 
 
