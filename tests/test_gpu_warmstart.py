@@ -47,3 +47,18 @@ def test_judge_on_a_single_case_context():
     kept, r_cold, _ = warmstart.judged_start(s, U)
     assert kept.tolist() == [False]
     assert np.array_equal(s.U, np.tile(s.freestream(), (m.nc, 1)))
+
+
+def test_saved_network_predicts_the_same_bits(tmp_path):
+    """save_net / load_net carry a network between processes: the loaded
+    one predicts exactly what the saved one does, and the training figures
+    come back with it."""
+    net = warmstart._mlp.MLP(warmstart.NIN, 256, 8, warmstart.NOUT)
+    net.set_params(*warmstart._init(warmstart.NIN, 256, 8, warmstart.NOUT, 77))
+    case = warmstart.family_cases(1, 13)[0]
+    x = warmstart.features(warmstart.family_mesh(case), case)
+    path = str(tmp_path / "net.npz")
+    warmstart.save_net(path, net, 10, 9, len(x), 3, [0.5, 0.25, 0.125], 1.5)
+    loaded, losses, t_train, meta = warmstart.load_net(path)
+    assert meta == (10, 9, len(x), 3) and losses == [0.5, 0.25, 0.125] and t_train == 1.5
+    assert np.array_equal(net.predict(x), loaded.predict(x))
