@@ -395,11 +395,11 @@ __device__ __forceinline__ double lusgs_ra(const double *Ua, double n0, double n
 // the SST flux, all area-weighted and oriented out of the left cell; the
 // sweeps' r_A of both sides go to their cell slots.
 // No occupancy attribute: held to 96 VGPRs (5 waves per SIMD) the SST face
-// kernel spills 10 VGPRs and runs 198 us instead of 179 us per launch on the
-// 1 M-cell mesh; at its own 110 VGPRs it runs 4 waves without spilling.
+// kernel spills 5 VGPRs and runs 179 us instead of 173 us per launch on the
+// 1 M-cell mesh; at its own 104 VGPRs it runs 4 waves without spilling.
 // ---------------------------------------------------------------------------
 template <int FLOW, typename T, bool B, typename Out>
-__device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, const Consts &k_, int f, Out out) {
+__device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, const Consts &k_, int f, Out &out) {
   using F = Flow<FLOW>;
   const int nc = m.nc, nf = m.nf;
   const T Gamma = T(USG_GAMMA);
@@ -444,8 +444,8 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
       mprR = cell_mpr<F::sst>(s, k, Rr);
       dist = m.fdist[f];
     }
-    s.rA[(size_t)m.fslot[f] * nc + L] = lusgs_ra<F::visc>(URd, m.fnx[f], m.fny[f], mprL, dist);
-    s.rA[(size_t)m.fslot[nf + f] * nc + Rr] = lusgs_ra<F::visc>(ULd, m.fnx[f], m.fny[f], mprR, dist);
+    out.ra((size_t)m.fslot[f] * nc + L, lusgs_ra<F::visc>(URd, m.fnx[f], m.fny[f], mprL, dist),
+           (size_t)m.fslot[nf + f] * nc + Rr, lusgs_ra<F::visc>(ULd, m.fnx[f], m.fny[f], mprR, dist));
   }
   if (inner)
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
@@ -589,13 +589,25 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
 }
 
 
-// The face record in global memory, quantity q at FF[q*nf + f] (one array
-// per quantity, so a wave's store of one quantity is contiguous).
+// Quantities of the face record by flow: Euler 5, laminar 10, SST 12.
+template <int FLOW>
+__host__ __device__ constexpr int face_q() { return FLOW == USG_SST ? 12 : (FLOW == USG_LAMINAR ? 10 : 5); }
+
+// The face record held in registers until face_terms has made every load:
+// a store issued among the loads would make the compiler reload what it may
+// alias and keep later loads behind it.  Then quantity q goes to FF[q*nf + f]
+// (one array per quantity, so a wave's store of one quantity is contiguous)
+// and the two r_A values to their cell slots.
 template <typename T>
-struct FaceOutGlobal {
-  T *FF;
-  int nf;
-  __device__ __forceinline__ void operator()(int q, T v) const { FF[(size_t)q * nf] = v; }
+struct FaceOutRegs {
+  T v[12];
+  size_t ia = 0, ib = 0;
+  double ra_a = 0.0, ra_b = 0.0;
+  bool has_ra = false;
+  __device__ __forceinline__ void operator()(int q, T x) { v[q] = x; }
+  __device__ __forceinline__ void ra(size_t a, double va, size_t b, double vb) {
+    ia = a, ra_a = va, ib = b, ra_b = vb, has_ra = true;
+  }
 };
 
 template <int FLOW, typename T, bool B>
@@ -603,7 +615,14 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
                                                 const int *__restrict__ items, const XSlice sl) {
   int f;
   if (!xcd_item(items, sl, f)) return;
-  face_terms<FLOW, T, B>(m, s, k_, f, FaceOutGlobal<T>{pick<T>(s.FF, s.FFf) + f, m.nf});
+  FaceOutRegs<T> o;
+  face_terms<FLOW, T, B>(m, s, k_, f, o);
+  T *FF = pick<T>(s.FF, s.FFf) + f;
+  for (int q = 0; q < face_q<FLOW>(); q++) FF[(size_t)q * m.nf] = o.v[q];
+  if (o.has_ra) {
+    s.rA[o.ia] = o.ra_a;
+    s.rA[o.ib] = o.ra_b;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -613,24 +632,45 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
 //   fwd_store: dW1 = (-R - sum_lower DF) / D        (LUSGS_Integrator.cpp:492-523)
 //   bwd_store: dW = dW1 - sum_upper DF / D, W += dW, U = prim(W), SST update
 //              (LUSGS_Integrator.cpp:242-255,526-561, SSTKWStrategy.h:194-223)
+// Each makes all of its loads before its first store: on gfx950 one counter
+// tracks a wave's loads and stores in issue order, so a load issued after a
+// store is waited for only together with that store, and the compiler may
+// not move a load above a store it cannot prove disjoint.
 // ---------------------------------------------------------------------------
+// fwd_store from the cell's residual R, tR and SST diagonal tD already in
+// registers.
 template <int FLOW>
-__device__ __forceinline__ void fwd_store(const DevMesh &m, const DevState &s, int c, const double *dFi,
-                                          const double *tFi, double diag, double *dW1, double *tdW1) {
+__device__ __forceinline__ void fwd_store_with(const DevMesh &m, const DevState &s, int c, const double *R,
+                                               const double *tR, const double *tD, const double *dFi,
+                                               const double *tFi, double diag, double *dW1, double *tdW1) {
   using F = Flow<FLOW>;
   const int nc = m.nc;
   // overset hole and receptor cells take no increment (their lower
   // neighbours then see dW1 = 0 across the overset boundary)
   const bool fz = frozen_cell(m, c);
-  for (int j = 0; j < 4; j++) {
-    dW1[j] = fz ? 0.0 : (-s.R[j * nc + c] - dFi[j]) / diag;
-    s.dW1[j * nc + c] = dW1[j];
+  for (int j = 0; j < 4; j++) dW1[j] = fz ? 0.0 : (-R[j] - dFi[j]) / diag;
+  if (F::sst)
+    for (int q = 0; q < 2; q++) tdW1[q] = fz ? 0.0 : (-tR[q] - tFi[q]) / tD[q];
+  store4(s.dW1, nc, c, dW1);
+  if (F::sst) {
+    s.tdW1[c] = tdW1[0];
+    s.tdW1[nc + c] = tdW1[1];
   }
+}
+
+template <int FLOW>
+__device__ __forceinline__ void fwd_store(const DevMesh &m, const DevState &s, int c, const double *dFi,
+                                          const double *tFi, double diag, double *dW1, double *tdW1) {
+  using F = Flow<FLOW>;
+  const int nc = m.nc;
+  double R[4], tR[2] = {0.0, 0.0}, tD[2] = {1.0, 1.0};
+  load4(s.R, nc, c, R);
   if (F::sst)
     for (int q = 0; q < 2; q++) {
-      tdW1[q] = fz ? 0.0 : (-s.tR[q * nc + c] - tFi[q]) / s.tD[q * nc + c];
-      s.tdW1[q * nc + c] = tdW1[q];
+      tR[q] = s.tR[q * nc + c];
+      tD[q] = s.tD[q * nc + c];
     }
+  fwd_store_with<FLOW>(m, s, c, R, tR, tD, dFi, tFi, diag, dW1, tdW1);
 }
 
 // A cell that is not updated this iteration carries its state into the next
@@ -639,18 +679,27 @@ template <int FLOW>
 __device__ __forceinline__ void carry_state(const DevMesh &m, const DevState &s, int c) {
   using F = Flow<FLOW>;
   const int nc = m.nc;
-  for (int j = 0; j < 4; j++) {
-    s.dW[j * nc + c] = 0.0;
-    s.Wn[j * nc + c] = s.W[j * nc + c];
-    s.Un[j * nc + c] = s.U[j * nc + c];
+  double W[4], U[4], tw[2] = {0.0, 0.0}, tu[2] = {0.0, 0.0}, mt = 0.0;
+  load4(s.W, nc, c, W);
+  load4(s.U, nc, c, U);
+  if (F::sst) {
+    for (int q = 0; q < 2; q++) {
+      tw[q] = s.tW[q * nc + c];
+      tu[q] = s.tU[q * nc + c];
+    }
+    mt = s.mut[c];
   }
+  const double zero[4] = {0.0, 0.0, 0.0, 0.0};
+  store4(s.dW, nc, c, zero);
+  store4(s.Wn, nc, c, W);
+  store4(s.Un, nc, c, U);
   if (F::sst) {
     for (int q = 0; q < 2; q++) {
       s.tdW[q * nc + c] = 0.0;
-      s.tWn[q * nc + c] = s.tW[q * nc + c];
-      s.tUn[q * nc + c] = s.tU[q * nc + c];
+      s.tWn[q * nc + c] = tw[q];
+      s.tUn[q * nc + c] = tu[q];
     }
-    s.mutn[c] = s.mut[c];
+    s.mutn[c] = mt;
   }
 }
 
@@ -672,21 +721,33 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
     carry_state<FLOW>(m, s, c);
     return;
   }
+  double W[4], tD[2] = {1.0, 1.0}, tW[2] = {0.0, 0.0}, g5 = 0.0, g6 = 0.0, f2 = 0.0;
+  load4(s.W, nc, c, W);
+  if (F::sst) {
+    for (int q = 0; q < 2; q++) {
+      tD[q] = s.tD[q * nc + c];
+      tW[q] = s.tW[q * nc + c];
+    }
+    const T *G = pickc<T>(s.grad, s.gradf);
+    g5 = double(G[5 * nc + c]);
+    g6 = double(G[6 * nc + c]);
+    f2 = double(pickc<T>(s.F2, s.F2f)[c]);
+  }
   double dW[4], Wn[4], Un[4];
   for (int j = 0; j < 4; j++) {
     dW[j] = dW1[j] - dFi[j] / diag;
-    s.dW[j * nc + c] = dW[j];
-    Wn[j] = s.W[j * nc + c] + dW[j];
+    Wn[j] = W[j] + dW[j];
   }
   prim_of(Wn, Un);
+  store4(s.dW, nc, c, dW);
   store4(s.Wn, nc, c, Wn);
   store4(s.Un, nc, c, Un);
   if (F::sst) {
     double tdw[2], tw[2];
     for (int q = 0; q < 2; q++) {
-      tdw[q] = tdW1[q] - tFi[q] / s.tD[q * nc + c];
+      tdw[q] = tdW1[q] - tFi[q] / tD[q];
       s.tdW[q * nc + c] = tdw[q];
-      tw[q] = s.tW[q * nc + c] + tdw[q];
+      tw[q] = tW[q] + tdw[q];
     }
     double rho = Un[0];
     double tu[2] = {tw[0] / rho, tw[1] / rho};
@@ -703,9 +764,8 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
     s.tWn[nc + c] = tw[1];
     s.tUn[c] = tu[0];
     s.tUn[nc + c] = tu[1];
-    const T *G = pickc<T>(s.grad, s.gradf);
-    double curl = fabs(double(G[6 * nc + c]) - double(G[5 * nc + c]));
-    double h = smax(SST_A1 * tw[1] / rho, double(pickc<T>(s.F2, s.F2f)[c]) * curl);
+    double curl = fabs(g6 - g5);
+    double h = smax(SST_A1 * tw[1] / rho, f2 * curl);
     s.mutn[c] = SST_A1 * tw[0] / h;
   }
 }
@@ -762,23 +822,20 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
   }
   double lamc = 0, lamv = 0;
   for (int i = 0; i < ns; i++) lamc += in(i, fs[i], 4);
-  s.lamc[c] = lamc;
-  if (F::visc) {
+  if (F::visc)
     for (int i = 0; i < ns; i++) lamv += in(i, fs[i], 9) / V;
-    s.lamv[c] = lamv;
-  }
-  store4(s.R, nc, c, Rc);
   const double OMEGAN = 1.5;
   double dtc;
   if (F::sst)
     dtc = k.cfl * V / (lamc + 4.0 * lamv);
   else
     dtc = k.cfl * V / (lamc);
-  s.dt[c] = dtc;
   double diag = V / dtc + 0.5 * OMEGAN * lamc;
   if (F::visc) diag += lamv;
   if (k.dts[0] != 0.0) diag += k.dts[0] * V;
-  s.D[c] = diag;
+  // every load of the cell is made before its first store (a store among
+  // the loads would hold the later loads behind it)
+  double trc[2] = {0.0, 0.0}, tdiag[2] = {0.0, 0.0};
   if (F::sst) {
     double tUc[2] = {s.tU[c], s.tU[nc + c]};
     const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
@@ -798,7 +855,8 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
     double kk = tUc[0], w = tUc[1];
     double Q0 = P - SST_BSTAR * rho * kk * w;
     double Q1 = Cw * rho * P / mutc - beta * rho * w * w + 2 * (1 - F1c) * rho * SST_SW2 / w * rdiv;
-    double trc[2] = {-(Q0 * V), -(Q1 * V)};
+    trc[0] = -(Q0 * V);
+    trc[1] = -(Q1 * V);
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
       if (e >= 0) {
@@ -809,21 +867,31 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
         trc[1] -= in(i, e, 11);
       }
     }
-    s.tR[c] = trc[0];
-    s.tR[nc + c] = trc[1];
     // SSTkomegaModel::SolveDiag
     double diagt = V / dtc + 0.5 * OMEGAN * lamc + lamv;
-    s.tD[c] = diagt + 2.0 * SST_BSTAR * w * V;
+    tdiag[0] = diagt + 2.0 * SST_BSTAR * w * V;
     double gradVal = rdiv;
-    s.tD[nc + c] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - F1c) * gradVal / (w * w));
+    tdiag[1] = diagt + V * fabs(2.0 * beta * w + 2.0 * SST_SW2 * (1 - F1c) * gradVal / (w * w));
+  }
+  const unsigned char wf = m.wflag[c];
+  s.lamc[c] = lamc;
+  if (F::visc) s.lamv[c] = lamv;
+  store4(s.R, nc, c, Rc);
+  s.dt[c] = dtc;
+  s.D[c] = diag;
+  if (F::sst) {
+    s.tR[c] = trc[0];
+    s.tR[nc + c] = trc[1];
+    s.tD[c] = tdiag[0];
+    s.tD[nc + c] = tdiag[1];
   }
   // cells without lower neighbours (forward wave 0): dW1 = (-R - 0) / D here,
   // and the backward update too for cells with no upper neighbour either
-  if (m.wflag[c] & 1) {
+  if (wf & 1) {
     const double zero[4] = {0.0, 0.0, 0.0, 0.0};
     double dW1[4], tdW1[2];
-    fwd_store<FLOW>(m, s, c, zero, zero, diag, dW1, tdW1);
-    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
+    fwd_store_with<FLOW>(m, s, c, Rc, trc, tdiag, zero, zero, diag, dW1, tdW1);
+    if (wf & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   }
 }
 
@@ -860,7 +928,12 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
 struct FaceOutLds {
   double *L;
   int stride, i;
+  double *rA;
   __device__ __forceinline__ void operator()(int q, double v) const { L[q * stride + i] = v; }
+  __device__ __forceinline__ void ra(size_t a, double va, size_t b, double vb) const {
+    rA[a] = va;
+    rA[b] = vb;
+  }
 };
 struct FaceInLds {
   const double *L;
@@ -876,8 +949,10 @@ __global__ __launch_bounds__(kBlock) void k_facecell(const DevMesh m, const DevS
                                                     const unsigned short *__restrict__ lfs, int stride) {
   extern __shared__ double lds[];
   const int t = blockIdx.x, f0 = tfoff[t], f1 = tfoff[t + 1];
-  for (int i = f0 + (int)threadIdx.x; i < f1; i += (int)blockDim.x)
-    face_terms<FLOW, double, B>(m, s, k_, tface[i], FaceOutLds{lds, stride, i - f0});
+  for (int i = f0 + (int)threadIdx.x; i < f1; i += (int)blockDim.x) {
+    FaceOutLds o{lds, stride, i - f0, s.rA};
+    face_terms<FLOW, double, B>(m, s, k_, tface[i], o);
+  }
   __syncthreads();
   int c;
   if (!xcd_item(items, sl, c)) return;
@@ -1073,9 +1148,10 @@ __device__ __forceinline__ void sweep_tail_cell(const DevMesh &m, const DevState
   const double diag = s.D[c];
   const double zero[4] = {0.0, 0.0, 0.0, 0.0};
   if (FWD) {
+    const bool last = m.wflag[c] & 2;  // loaded before fwd_store's stores
     double dW1[4], tdW1[2];
     fwd_store<FLOW>(m, s, c, dFi, tFi, diag, dW1, tdW1);
-    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
+    if (last) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   } else {
     double dW1[4], tdW1[2] = {0.0, 0.0};
     for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
@@ -1108,6 +1184,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
   const Consts k = case_consts<B>(m, k_, c);
   const int nc = m.nc;
   const double diag = s.D[c];
+  const bool last = FWD && (m.wflag[c] & 2);  // loaded before the first store
   double dFi[4] = {0, 0, 0, 0};
   double tFi[2] = {0, 0};
   df_sum<FLOW, FWD>(m, s, c, dFi, tFi);
@@ -1116,7 +1193,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     double dW1[4], tdW1[2];
     fwd_store<FLOW>(m, s, c, dFi, tFi, diag, dW1, tdW1);
     // a cell without upper neighbours (backward wave 0) ends here
-    if (m.wflag[c] & 2) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
+    if (last) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   } else {
     double dW1[4], tdW1[2] = {0.0, 0.0};
     for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
@@ -1617,10 +1694,6 @@ static int xgrid(const XSlice &sl, int bs) {
   return kXcd * std::max(1, (mx + bs - 1) / bs);
 }
 
-// quantities of a face record: flux 4 + spectral 1 (+ viscous flux 4 +
-// viscous spectral 1) (+ SST 2)
-template <int FLOW>
-constexpr int face_q() { return FLOW == USG_SST ? 12 : (FLOW == USG_LAMINAR ? 10 : 5); }
 
 constexpr size_t kMaxTileLds = 64 * 1024;
 
