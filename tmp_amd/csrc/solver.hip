@@ -372,6 +372,17 @@ __device__ __forceinline__ double cell_mpr(const DevState &s, const Consts &k, i
   return mprc;
 }
 
+// max(4/(3 rho), gamma/rho) of the viscous spectral radius
+// (ViscousFlow2D.cpp:243-249, TopologyHolderStrategy.cpp:217-323).  For
+// rho > 0 the second term is the larger (4/3 against gamma = 1.4, far beyond
+// rounding, overflow and underflow included), so only it is divided out;
+// any other rho takes the whole expression.  The bits are the same.
+template <typename T>
+__device__ __forceinline__ T visc_spectral_max(T rho) {
+  const T Gamma = T(USG_GAMMA);
+  return rho > T(0.0) ? Gamma / rho : smax(T(4) / (T(3.0) * rho), Gamma / rho);
+}
+
 // r_A = omega (|V_j . n| + c_j) (+ max(4/(3 rho_j), gamma/rho_j) (mu/Pr)_i /
 // |x_j - x_i| if viscous) of cell i receiving from neighbour j with state
 // Ua (SolveDF, TopologyHolderStrategy.cpp:217-323)
@@ -381,7 +392,7 @@ __device__ __forceinline__ double lusgs_ra(const double *Ua, double n0, double n
   double Vn = dot2(Ua[2], Ua[3], n0, n1);
   double cs = sqrt(Gamma * Ua[1] / Ua[0]);
   double rA = OMEGAN * (fabs(Vn) + cs);
-  if (VISC) rA += smax(4 / (3 * Ua[0]), Gamma / Ua[0]) * (mprc) / dist;
+  if (VISC) rA += visc_spectral_max(Ua[0]) * (mprc) / dist;
   return rA;
 }
 
@@ -521,7 +532,7 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   for (int j = 0; j < 4; j++) out(5 + j, Fv[j]);
   // viscous spectral radius term (ViscousFlow2D.cpp:243-249); each cell
   // divides it by its own volume
-  out(9, smax(T(4) / (T(3.0) * Ue[0]), Gamma / Ue[0]) * mpr * A * A);
+  out(9, visc_spectral_max(Ue[0]) * mpr * A * A);
   if (!F::sst) return;
   // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
   const T Vn = dot2(Ue[2], Ue[3], n0, n1);
