@@ -5,6 +5,7 @@ kernel call count, total and average duration, share of GPU time.
     python -m tmp_amd.profsummary --pmc <results.db>
     python -m tmp_amd.profsummary --traffic KERNEL --cells NC --copy-bytes B \
         <fetch.db> <write.db> <copy_fetch.db> <copy_write.db>
+    python -m tmp_amd.profsummary --sq <trace.db> <sq_pmc.db> --kernels K...
 
 The traffic mode turns FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of one
 kernel into HBM bytes per launch, each counter scaled by the factor that makes
@@ -102,6 +103,34 @@ def hbm(trace_db: str, dbs: list, copy_bytes: int, kernels: list, command: str) 
         print(f"| `{n[:70]}` | {avg[n]:.1f} | {rb / 1e6:.1f} | {wb / 1e6:.1f} | {gbs:.0f} | {gbs / 8000:.2f} |")
 
 
+def sq(trace_db: str, pmc_db: str, kernels: list, command: str, simds: int = 1024, ghz: float = 2.4) -> None:
+    """Per kernel, per wave: VALU instructions, vector memory reads and
+    writes, the wave's lifetime and the share of it spent waiting to issue
+    (SQ_WAVES, SQ_INSTS_VALU, SQ_INSTS_VMEM_RD/WR, SQ_WAVE_CYCLES,
+    SQ_WAIT_INST_ANY); and the share of the chip's VALU issue cycles the
+    kernel's VALU instructions take over its average duration (4 cycles per
+    wave64 instruction on a 16-lane SIMD, `simds` SIMDs at `ghz`)."""
+    times = {name: avg for name, _, _, avg, _ in summary(trace_db)}
+    cs = counters(pmc_db)
+    print(f"# SQ instruction and wait counters per kernel\n")
+    if command:
+        print(f"Commands: {command}\n")
+    print("| kernel | waves | VALU / wave | loads / wave | stores / wave | wave cycles / wave | waiting | "
+          f"VALU issue share ({simds} SIMDs, {ghz} GHz) |")
+    print("|---|---:|---:|---:|---:|---:|---:|---:|")
+    for kn in kernels:
+        hits = [(k, v) for k, v in cs.items() if kn in k]
+        if len(hits) != 1:
+            raise SystemExit(f"{pmc_db}: {len(hits)} kernels match {kn!r}")
+        k, v = hits[0]
+        w = v["SQ_WAVES"]
+        tk = [t for name, t in times.items() if name == k]
+        share = (w * v["SQ_INSTS_VALU"] / w * 4) / (simds * tk[0] * 1e-6 * ghz * 1e9) if tk else float("nan")
+        print(f"| `{k[:60]}` | {w:.0f} | {v['SQ_INSTS_VALU'] / w:.0f} | {v['SQ_INSTS_VMEM_RD'] / w:.1f} | "
+              f"{v['SQ_INSTS_VMEM_WR'] / w:.1f} | {v['SQ_WAVE_CYCLES'] / w:.0f} | "
+              f"{v['SQ_WAIT_INST_ANY'] / v['SQ_WAVE_CYCLES']:.2f} | {share:.2f} |")
+
+
 def mfma(trace_db: str, pmc_db: str, command: str, peak_tflops: float = 2500.0) -> None:
     """Per MFMA kernel: BF16 FLOPs per dispatch (SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512),
     average duration, TFLOP/s and its fraction of the dense BF16 peak."""
@@ -131,6 +160,7 @@ def main():
     ap.add_argument("--kernels", nargs="*", default=["k_face", "k_cell", "k_grad", "k_sweep<2, true",
                                                      "k_sweep<2, false", "k_norm_partial"])
     ap.add_argument("--mfma", metavar="TRACE_DB", help="MFMA utilisation: db = the --pmc db")
+    ap.add_argument("--sq", metavar="TRACE_DB", help="SQ counters per wave: db = the SQ --pmc db")
     ap.add_argument("--traffic", metavar="KERNEL")
     ap.add_argument("--cells", type=int, default=0)
     ap.add_argument("--copy-bytes", type=int, default=0)
@@ -143,6 +173,9 @@ def main():
         return
     if a.mfma:
         mfma(a.mfma, a.db[0], a.command)
+        return
+    if a.sq:
+        sq(a.sq, a.db[0], a.kernels, a.command)
         return
     if a.traffic:
         traffic(a.traffic, a.cells, a.copy_bytes, a.db, a.command)
