@@ -993,7 +993,8 @@ __device__ void norm_ring_block(const double *partial, const int *boff, double *
 // per-block partial sums (kind 1: n = one per norm block) or its ring sum
 // (kind 2: n = 1), the same code on the same cells as k_norm_partial and
 // k_norm_ring, which they replace.  R is not written by the sweeps, and the
-// ring sum runs in the launch after the partial sums.
+// ring sum runs in the sweep launch after the partial sums (forward or
+// backward).
 struct NormBlocks {
   int kind = 0, nsweep = 0, cap = 0;
   const double *R = nullptr;
@@ -1840,8 +1841,9 @@ static void launch_residual_any(usg_ctx *c) {
 }
 
 // A sweep's last wave runs as a tail of the wave before it when it holds
-// at most kTailMax cells and at least min_waves waves exist (the forward
-// sweep keeps two launches of its own for the norm blocks: 4; backward: 3).
+// at most kTailMax cells and at least min_waves waves exist (3: wave 0 runs
+// inside k_cell or the forward waves, so the tail needs a launch of its own
+// wave to ride on).
 constexpr int kTailMax = 8192;
 static bool sweep_tail(const std::vector<int> &poff, int min_waves) {
   const int nw = (int)poff.size() / (kXcd + 1);
@@ -1859,25 +1861,27 @@ static SweepTail tail_of(const std::vector<int> &poff, int bs) {
   return tl;
 }
 
-// With fuse_norm, the first two forward launches also carry the residual
-// norm of this iteration into the ring (NormBlocks).
+// Sweep launches per iteration: each sweep's waves after wave 0, less the
+// small last wave when it runs as a tail.
+static int sweep_launches(const usg_ctx *c) {
+  const int nfw = (int)c->hm.fwd_poff.size() / (kXcd + 1), nbw = (int)c->hm.bwd_poff.size() / (kXcd + 1);
+  return std::max(0, nfw - 1 - (c->fwd_tail ? 1 : 0)) + std::max(0, nbw - 1 - (c->bwd_tail ? 1 : 0));
+}
+
+// With fuse_norm, the first two sweep launches of the iteration also carry
+// the residual norm into the ring (NormBlocks): the partial sums on the
+// first, the ring sum on the second (forward or backward: neither writes R).
 template <int FLOW, typename T, bool B>
 static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
   const HostMesh &h = c->hm;
   const int bs = c->blk[USG_K_SWEEP];
-  // wave 0 of each sweep is fused into k_cell / the forward waves (wflag);
-  // a small last wave runs as a tail of the wave before it
-  const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
-  for (int w = 1; w < nw - (c->fwd_tail ? 1 : 0); w++) {
-    XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
-    const int nmain = xgrid(sl, bs);
-    const SweepTail tl = c->fwd_tail && w == nw - 2 ? tail_of(h.fwd_poff, bs) : SweepTail{};
-    const int ntail = tl.nblk;
+  int launch = 0;
+  auto norm_blocks = [&](int nsweep, int &extra) {
     NormBlocks nb;
-    int extra = 0;
-    if (fuse_norm && w <= 2) {
-      nb.kind = w;
-      nb.nsweep = nmain + ntail;
+    extra = 0;
+    if (fuse_norm && launch < 2) {
+      nb.kind = launch + 1;
+      nb.nsweep = nsweep;
       nb.cap = kRing;
       nb.R = c->ds.R;
       nb.b0 = c->d_nb0;
@@ -1886,19 +1890,34 @@ static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
       nb.partial = c->d_partial;
       nb.ring = c->d_ring;
       nb.ctr = c->d_ctr;
-      extra = w == 1 ? c->nnb : 1;
+      extra = nb.kind == 1 ? c->nnb : 1;
     }
+    launch++;
+    return nb;
+  };
+  // wave 0 of each sweep is fused into k_cell / the forward waves (wflag);
+  // a small last wave runs as a tail of the wave before it
+  const int nw = (int)h.fwd_poff.size() / (kXcd + 1);
+  for (int w = 1; w < nw - (c->fwd_tail ? 1 : 0); w++) {
+    XSlice sl = slice(&h.fwd_poff[(size_t)w * (kXcd + 1)]);
+    const int nmain = xgrid(sl, bs);
+    const SweepTail tl = c->fwd_tail && w == nw - 2 ? tail_of(h.fwd_poff, bs) : SweepTail{};
+    const int ntail = tl.nblk;
+    int extra = 0;
+    const NormBlocks nb = norm_blocks(nmain + ntail, extra);
     hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(nmain + ntail + extra), dim3(bs), 0, c->stream, c->dm,
                        c->ds, c->k, c->d_fwd, sl, nb, tl);
   }
-  const int nb = (int)h.bwd_poff.size() / (kXcd + 1);
-  for (int w = 1; w < nb - (c->bwd_tail ? 1 : 0); w++) {
+  const int nbw = (int)h.bwd_poff.size() / (kXcd + 1);
+  for (int w = 1; w < nbw - (c->bwd_tail ? 1 : 0); w++) {
     XSlice sl = slice(&h.bwd_poff[(size_t)w * (kXcd + 1)]);
     const int nmain = xgrid(sl, bs);
-    const SweepTail tl = c->bwd_tail && w == nb - 2 ? tail_of(h.bwd_poff, bs) : SweepTail{};
+    const SweepTail tl = c->bwd_tail && w == nbw - 2 ? tail_of(h.bwd_poff, bs) : SweepTail{};
     const int ntail = tl.nblk;
-    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(nmain + ntail), dim3(bs), 0, c->stream, c->dm, c->ds,
-                       c->k, c->d_bwd, sl, NormBlocks{}, tl);
+    int extra = 0;
+    const NormBlocks nb = norm_blocks(nmain + ntail, extra);
+    hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(nmain + ntail + extra), dim3(bs), 0, c->stream, c->dm,
+                       c->ds, c->k, c->d_bwd, sl, nb, tl);
   }
 }
 
@@ -2148,7 +2167,7 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
       for (int i = h.fwd_poff[0]; i < h.fwd_poff[kXcd]; i++) wflag[h.fwd_list[i]] |= 1;
     if (!h.bwd_poff.empty())
       for (int i = h.bwd_poff[0]; i < h.bwd_poff[kXcd]; i++) wflag[h.bwd_list[i]] |= 2;
-    c->fwd_tail = sweep_tail(h.fwd_poff, 4);
+    c->fwd_tail = sweep_tail(h.fwd_poff, 3);
     c->bwd_tail = sweep_tail(h.bwd_poff, 3);
     for (int d = 0; d < 2; d++) {
       const std::vector<int> &poff = d ? h.bwd_poff : h.fwd_poff;
@@ -2428,10 +2447,9 @@ static void launch_norm_ring(usg_ctx *c) {
 }
 
 // The norm rides on the sweep launches for a single case whose sweeps use
-// kBlock-thread blocks and have at least two forward launches.
+// kBlock-thread blocks and make at least two launches.
 static bool fuse_norm(const usg_ctx *c) {
-  const int nw = (int)c->hm.fwd_poff.size() / (kXcd + 1);
-  return c->ncase == 1 && c->blk[USG_K_SWEEP] == kBlock && nw >= 3;
+  return c->ncase == 1 && c->blk[USG_K_SWEEP] == kBlock && sweep_launches(c) >= 2;
 }
 
 // One LU-SGS iteration (LUSGSIntegrator::UpdateSerial): residual, norm into
