@@ -14,7 +14,8 @@ constexpr int kXcd = 8;                    // XCDs (and spatial parts) of an MI3
 
 // A work list split into kXcd parts: items[off[p] .. off[p+1]) belong to part
 // p.  Block b of a launch serves part b % kXcd (workgroups are dispatched to
-// the XCDs round-robin), chunk b / kXcd.
+// the XCDs round-robin), chunk b / kXcd.  A null list is the identity (item
+// i is i): the launch then saves the dependent load of its item.
 struct XSlice {
   int off[kXcd + 1];
 };
@@ -23,7 +24,7 @@ __device__ __forceinline__ bool xcd_item_at(const int *__restrict__ items, const
   const int part = b % kXcd;
   const int i = sl.off[part] + (b / kXcd) * (int)blockDim.x + (int)threadIdx.x;
   if (i >= sl.off[part + 1]) return false;
-  item = items[i];
+  item = items ? items[i] : i;
   return true;
 }
 __device__ __forceinline__ bool xcd_item(const int *__restrict__ items, const XSlice &sl, int &item) {

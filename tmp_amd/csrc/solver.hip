@@ -1124,7 +1124,7 @@ __device__ __forceinline__ void sweep_tail_cell(const DevMesh &m, const DevState
   const int i = tl.sl.off[part] + (b / kXcd) * ((int)blockDim.x >> 2) + ((int)threadIdx.x >> 2);
   if (i >= tl.sl.off[part + 1]) return;  // the four lanes of a cell leave together
   const int q = (int)threadIdx.x & 3;
-  const int c = items[i];
+  const int c = items ? items[i] : i;
   const int nc = m.nc;
   double t[4] = {0.0, 0.0, 0.0, 0.0}, tt[2] = {0.0, 0.0};
   int has = 0;
@@ -1632,6 +1632,11 @@ struct usg_ctx {
   // the small last wave of the forward / backward sweep runs as a tail of
   // the wave before it (SweepTail)
   bool fwd_tail = false, bwd_tail = false;
+  // the face launch and the sweep waves may take their items as indices (a
+  // null list): every face once in any order (single case), and sweep waves
+  // that are contiguous cell ranges (the colour-blocked numbering), whose
+  // parts are then contiguous sub-ranges
+  bool face_ident = false, sweep_ident = false;
   int tile_bs = 0, tile_stride = 0;
   int *d_tfoff = nullptr, *d_tface = nullptr;
   unsigned short *d_lfs = nullptr;
@@ -1794,7 +1799,7 @@ static void launch_residual(usg_ctx *c, hipEvent_t ea, hipEvent_t eb) {
   }
   if (ea) hipEventRecord(ea, c->stream);
   hipLaunchKernelGGL((k_face<FLOW, T, B>), dim3(xgrid(fs, bf)), dim3(bf), 0, c->stream, c->dm, c->ds, c->k,
-                     c->d_face_list, fs);
+                     c->face_ident ? nullptr : c->d_face_list, fs);
   if (eb) hipEventRecord(eb, c->stream);
   hipLaunchKernelGGL((k_cell<FLOW, T, B>), dim3(xgrid(cs, bc)), dim3(bc), 0, c->stream, c->dm, c->ds, c->k,
                      c->d_cell_list, cs);
@@ -1906,7 +1911,7 @@ static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
     int extra = 0;
     const NormBlocks nb = norm_blocks(nmain + ntail, extra);
     hipLaunchKernelGGL((k_sweep<FLOW, true, T, B>), dim3(nmain + ntail + extra), dim3(bs), 0, c->stream, c->dm,
-                       c->ds, c->k, c->d_fwd, sl, nb, tl);
+                       c->ds, c->k, c->sweep_ident ? nullptr : c->d_fwd, sl, nb, tl);
   }
   const int nbw = (int)h.bwd_poff.size() / (kXcd + 1);
   for (int w = 1; w < nbw - (c->bwd_tail ? 1 : 0); w++) {
@@ -1917,7 +1922,7 @@ static void launch_sweeps(usg_ctx *c, bool fuse_norm) {
     int extra = 0;
     const NormBlocks nb = norm_blocks(nmain + ntail, extra);
     hipLaunchKernelGGL((k_sweep<FLOW, false, T, B>), dim3(nmain + ntail + extra), dim3(bs), 0, c->stream, c->dm,
-                       c->ds, c->k, c->d_bwd, sl, nb, tl);
+                       c->ds, c->k, c->sweep_ident ? nullptr : c->d_bwd, sl, nb, tl);
   }
 }
 
@@ -2169,6 +2174,8 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
       for (int i = h.bwd_poff[0]; i < h.bwd_poff[kXcd]; i++) wflag[h.bwd_list[i]] |= 2;
     c->fwd_tail = sweep_tail(h.fwd_poff, 3);
     c->bwd_tail = sweep_tail(h.bwd_poff, 3);
+    c->face_ident = c->ncase == 1;
+    c->sweep_ident = c->ncase == 1 && h.use_ranges;
     for (int d = 0; d < 2; d++) {
       const std::vector<int> &poff = d ? h.bwd_poff : h.fwd_poff;
       const std::vector<int> &list = d ? h.bwd_list : h.fwd_list;
