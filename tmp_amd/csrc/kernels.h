@@ -98,8 +98,9 @@ struct DevState {
   double *dt;           // [nc] LU-SGS local time step
   double *dt_rk;        // [nc] RK5 time step, frozen at stage 0
   double *W0;           // [4][nc] RK5 stage origin
-  double *FF;           // face terms [q][nf] (q: flux 0-3, spectral 4, viscous flux 5-8,
-                        // viscous spectral 9, SST 10-11), oriented out of the left cell
+  double *FF;           // face terms (q: flux 0-3, spectral 4, viscous flux 5-8, viscous
+                        // spectral 9, SST 10-11), oriented out of the left cell, in
+                        // pairs [pair][nf][2] (rec_pair / rec_comp in the solver)
   double *D;            // [nc] LU-SGS diagonal
   double *tD;           // [2][nc] SST diagonal
   double *rA;           // [4][nc] LU-SGS r_A of each interior slot of a cell (from the

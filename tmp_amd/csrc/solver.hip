@@ -604,11 +604,39 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
 template <int FLOW>
 __host__ __device__ constexpr int face_q() { return FLOW == USG_SST ? 12 : (FLOW == USG_LAMINAR ? 10 : 5); }
 
+// The face record keeps its quantities in pairs, [pair][nf][2], so that a
+// face's two quantities come in one 16-byte access; the pairs follow the
+// order k_cell reads them in: convective flux (0,1) (2,3), viscous flux
+// (5,6) (7,8), spectral terms (4,9), SST (10,11).  Euler: (0,1) (2,3) (4,-).
+template <int FLOW>
+__host__ __device__ constexpr int rec_pairs() { return FLOW == USG_SST ? 6 : (FLOW == USG_LAMINAR ? 5 : 3); }
+template <int FLOW>
+__host__ __device__ constexpr int rec_pair(int q) {
+  return q < 4 ? q >> 1 : (q == 4 || q == 9) ? (FLOW == USG_EULER ? 2 : 4) : q < 9 ? 2 + ((q - 5) >> 1) : 5;
+}
+__host__ __device__ constexpr int rec_comp(int q) {
+  return q < 4 ? (q & 1) : q == 4 ? 0 : q == 9 ? 1 : q < 9 ? ((q - 5) & 1) : q - 10;
+}
+// The quantity held in component `comp` of pair p (-1: the Euler pad).
+template <int FLOW>
+__host__ __device__ constexpr int rec_q(int p, int comp) {
+  return p < 2 ? 2 * p + comp
+         : FLOW == USG_EULER ? (comp == 0 ? 4 : -1)
+         : p < 4 ? 5 + 2 * (p - 2) + comp
+         : p == 4 ? (comp == 0 ? 4 : 9) : 10 + comp;
+}
+// The pair of the convective flux, viscous flux, spectral terms and SST flux.
+template <int FLOW>
+struct RecPairs {
+  static constexpr int conv0 = 0, conv1 = 1, visc0 = 2, visc1 = 3;
+  static constexpr int lam = FLOW == USG_EULER ? 2 : 4, sst = 5;
+};
+
 // The face record held in registers until face_terms has made every load:
 // a store issued among the loads would make the compiler reload what it may
-// alias and keep later loads behind it.  Then quantity q goes to FF[q*nf + f]
-// (one array per quantity, so a wave's store of one quantity is contiguous)
-// and the two r_A values to their cell slots.
+// alias and keep later loads behind it.  Then each pair of quantities goes to
+// FF[pair][f] as one 16-byte store (contiguous over a wave) and the two r_A
+// values to their cell slots.
 template <typename T>
 struct FaceOutRegs {
   T v[12];
@@ -628,8 +656,17 @@ __global__ __launch_bounds__(kBlock) void k_face(const DevMesh m, const DevState
   if (!xcd_item(items, sl, f)) return;
   FaceOutRegs<T> o;
   face_terms<FLOW, T, B>(m, s, k_, f, o);
-  T *FF = pick<T>(s.FF, s.FFf) + f;
-  for (int q = 0; q < face_q<FLOW>(); q++) FF[(size_t)q * m.nf] = o.v[q];
+  using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+  T2 rec[rec_pairs<FLOW>()];
+  for (int p = 0; p < rec_pairs<FLOW>(); p++) rec[p].x = rec[p].y = T(0);
+  for (int q = 0; q < face_q<FLOW>(); q++) {
+    if (rec_comp(q) == 0)
+      rec[rec_pair<FLOW>(q)].x = o.v[q];
+    else
+      rec[rec_pair<FLOW>(q)].y = o.v[q];
+  }
+  T2 *FF = reinterpret_cast<T2 *>(pick<T>(s.FF, s.FFf));
+  for (int p = 0; p < rec_pairs<FLOW>(); p++) FF[(size_t)p * m.nf + f] = rec[p];
   if (o.has_ra) {
     s.rA[o.ia] = o.ra_a;
     s.rA[o.ib] = o.ra_b;
@@ -805,22 +842,27 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
   }
   // faces in ascending order, each added with this cell's sign (left +,
   // right -): the reference's scatter order.  Sums are FP64 in every mode.
+  using RP = RecPairs<FLOW>;
   double Rc[4] = {0, 0, 0, 0};
   for (int i = 0; i < ns; i++) {
     const int e = fs[i];
+    const double2 a = in.pair(i, e, RP::conv0), b = in.pair(i, e, RP::conv1);
+    const double v[4] = {a.x, a.y, b.x, b.y};
     if (e >= 0) {
-      for (int j = 0; j < 4; j++) Rc[j] += in(i, e, j);
+      for (int j = 0; j < 4; j++) Rc[j] += v[j];
     } else {
-      for (int j = 0; j < 4; j++) Rc[j] -= in(i, e, j);
+      for (int j = 0; j < 4; j++) Rc[j] -= v[j];
     }
   }
   if (F::visc) {
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
+      const double2 a = in.pair(i, e, RP::visc0), b = in.pair(i, e, RP::visc1);
+      const double v[4] = {a.x, a.y, b.x, b.y};
       if (e >= 0) {
-        for (int j = 0; j < 4; j++) Rc[j] -= in(i, e, 5 + j);
+        for (int j = 0; j < 4; j++) Rc[j] -= v[j];
       } else {
-        for (int j = 0; j < 4; j++) Rc[j] += in(i, e, 5 + j);
+        for (int j = 0; j < 4; j++) Rc[j] += v[j];
       }
     }
   }
@@ -832,9 +874,11 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
       Rc[j] += V * (k.dts[0] * s.W[j * nc + c] + k.dts[1] * s.Wt1[j * nc + c] + k.dts[2] * s.Wt2[j * nc + c]);
   }
   double lamc = 0, lamv = 0;
-  for (int i = 0; i < ns; i++) lamc += in(i, fs[i], 4);
-  if (F::visc)
-    for (int i = 0; i < ns; i++) lamv += in(i, fs[i], 9) / V;
+  for (int i = 0; i < ns; i++) {
+    const double2 l = in.pair(i, fs[i], RP::lam);
+    lamc += l.x;
+    if (F::visc) lamv += l.y / V;
+  }
   const double OMEGAN = 1.5;
   double dtc;
   if (F::sst)
@@ -870,12 +914,13 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
     trc[1] = -(Q1 * V);
     for (int i = 0; i < ns; i++) {
       const int e = fs[i];
+      const double2 t = in.pair(i, e, RP::sst);
       if (e >= 0) {
-        trc[0] += in(i, e, 10);
-        trc[1] += in(i, e, 11);
+        trc[0] += t.x;
+        trc[1] += t.y;
       } else {
-        trc[0] -= in(i, e, 10);
-        trc[1] -= in(i, e, 11);
+        trc[0] -= t.x;
+        trc[1] -= t.y;
       }
     }
     // SSTkomegaModel::SolveDiag
@@ -907,13 +952,16 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
 }
 
 
-// The face record in global memory (written by k_face), read by slot.
-template <typename T>
+// The face record in global memory (written by k_face), read by slot, one
+// pair of quantities per access.
+template <int FLOW, typename T>
 struct FaceInGlobal {
   const T *FF;
   int nf;
-  __device__ __forceinline__ double operator()(int, int e, int q) const {
-    return double(FF[(e >= 0 ? e : -e - 1) + (size_t)q * nf]);
+  __device__ __forceinline__ double2 pair(int, int e, int p) const {
+    using T2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+    const T2 v = reinterpret_cast<const T2 *>(FF)[(size_t)p * nf + (e >= 0 ? e : -e - 1)];
+    return make_double2(double(v.x), double(v.y));
   }
 };
 
@@ -922,7 +970,7 @@ __global__ __launch_bounds__(kBlock) void k_cell(const DevMesh m, const DevState
                                                 const int *__restrict__ items, const XSlice sl) {
   int c;
   if (!xcd_item(items, sl, c)) return;
-  cell_terms<FLOW, T, B>(m, s, k_, c, FaceInGlobal<T>{pickc<T>(s.FF, s.FFf), m.nf});
+  cell_terms<FLOW, T, B>(m, s, k_, c, FaceInGlobal<FLOW, T>{pickc<T>(s.FF, s.FFf), m.nf});
 }
 
 // ---------------------------------------------------------------------------
@@ -946,11 +994,15 @@ struct FaceOutLds {
     rA[b] = vb;
   }
 };
+template <int FLOW>
 struct FaceInLds {
   const double *L;
   int stride;
   const int *lf;  // local face index of each slot of the cell
-  __device__ __forceinline__ double operator()(int i, int, int q) const { return L[q * stride + lf[i]]; }
+  __device__ __forceinline__ double2 pair(int i, int, int p) const {
+    const int q0 = rec_q<FLOW>(p, 0), q1 = rec_q<FLOW>(p, 1);
+    return make_double2(L[q0 * stride + lf[i]], q1 >= 0 ? L[q1 * stride + lf[i]] : 0.0);
+  }
 };
 
 template <int FLOW, bool B>
@@ -969,7 +1021,7 @@ __global__ __launch_bounds__(kBlock) void k_facecell(const DevMesh m, const DevS
   if (!xcd_item(items, sl, c)) return;
   int lf[4];
   for (int q = 0; q < 4; q++) lf[q] = lfs[(size_t)q * m.nc + c];
-  cell_terms<FLOW, double, B>(m, s, k_, c, FaceInLds{lds, stride, lf});
+  cell_terms<FLOW, double, B>(m, s, k_, c, FaceInLds<FLOW>{lds, stride, lf});
 }
 
 // ---------------------------------------------------------------------------
@@ -1697,7 +1749,9 @@ static int grid(int n) { return (n + kBlock - 1) / kBlock; }
 
 // quantities per face: flux 4 + spectral 1 (+ viscous flux 4 + viscous
 // spectral 1) (+ SST 2)
-static int slot_q(int flow) { return flow == USG_SST ? 12 : (flow == USG_LAMINAR ? 10 : 5); }
+static int rec_pairs_of(int flow) {
+  return flow == USG_SST ? rec_pairs<USG_SST>() : flow == USG_LAMINAR ? rec_pairs<USG_LAMINAR>() : rec_pairs<USG_EULER>();
+}
 
 static XSlice slice(const int *off) {
   XSlice sl;
@@ -2231,11 +2285,11 @@ static int ctx_setup(usg_ctx *c, usg_ctx **out) {
   if (mixed) {
     CK(dalloc(c, &s.gradf, 8 * (size_t)nc));
     CK(dalloc(c, &s.phif, 4 * (size_t)nc));
-    CK(dalloc(c, &s.FFf, (size_t)slot_q(params->flow) * nf));
+    CK(dalloc(c, &s.FFf, (size_t)rec_pairs_of(params->flow) * 2 * nf));
   } else {
     CK(dalloc(c, &s.grad, 8 * (size_t)nc));
     CK(dalloc(c, &s.phi, 4 * (size_t)nc));
-    CK(dalloc(c, &s.FF, (size_t)slot_q(params->flow) * nf));
+    CK(dalloc(c, &s.FF, (size_t)rec_pairs_of(params->flow) * 2 * nf));
   }
   CK(dalloc(c, &s.lamc, nc));
   CK(dalloc(c, &s.R, 4 * (size_t)nc));
