@@ -88,11 +88,11 @@ __device__ __forceinline__ bool frozen_cell(const DevMesh &m, int c) { return m.
 struct DevState {
   double *U, *W;        // current primitive / conservative
   double *Un, *Wn;      // next iterate (double-buffered update)
-  double *grad;         // [8][nc] d(var)/dx, d(var)/dy interleaved per var: (2v+d)*nc
-  double *gradT;        // [2][nc]
+  double *grad;         // [4][nc][2]: (d(var)/dx, d(var)/dy) of var v in pair v (pidx(2v+d))
+  double *gradT;        // [1][nc][2]
   double *lamc, *lamv;  // [nc]
   double *mu;           // [nc] laminar viscosity at the cell
-  double *phi;          // [4][nc]
+  double *phi;          // [2][nc][2] (pidx(var))
   double *R;            // [4][nc]
   double *dW1, *dW;     // [4][nc]
   double *dt;           // [nc] LU-SGS local time step
@@ -107,7 +107,7 @@ struct DevState {
                         // neighbour across it), written by the face kernel
   // SST
   double *tU, *tW, *tUn, *tWn;  // [2][nc]
-  double *tgrad;                // [4][nc]: (2q+d)*nc
+  double *tgrad;                // [2][nc][2]: (d/dx, d/dy) of k, omega (pidx(2q+d))
   double *F1, *F2, *mut, *mutn; // [nc]
   double *tR, *tdW1, *tdW;      // [2][nc]
   int *err;                     // [2] invariant violation counters
@@ -136,6 +136,27 @@ __host__ __device__ __forceinline__ const T *pickc(const double *d, const float 
     return d;
   else
     return f;
+}
+
+// Gradient-like cell arrays (grad, phi, gradT, tgrad and their FP32
+// copies) keep their components in pairs, [k/2][nc][2]: component k of cell
+// c at ((k/2)*nc + c)*2 + k%2.  A gather of a pair -- d/dx and d/dy of one
+// variable, or two limiter values -- is then one 16-byte access.
+template <typename T>
+using Pair = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+template <typename T>
+__device__ __forceinline__ Pair<T> ld2(const T *a, int p, int nc, int c) {
+  return reinterpret_cast<const Pair<T> *>(a)[(size_t)p * nc + c];
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *a, int p, int nc, int c, T x, T y) {
+  Pair<T> v;
+  v.x = x;
+  v.y = y;
+  reinterpret_cast<Pair<T> *>(a)[(size_t)p * nc + c] = v;
+}
+__host__ __device__ __forceinline__ size_t pidx(int k, int nc, int c) {
+  return ((size_t)(k >> 1) * nc + c) * 2 + (k & 1);
 }
 
 struct Consts {

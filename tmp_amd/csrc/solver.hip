@@ -282,13 +282,12 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   const T Vt = T(V);
   T *G = pick<T>(s.grad, s.gradf);
   for (int j = 0; j < 8; j++) g[j] = g[j] / Vt;
-  for (int j = 0; j < 8; j++) G[j * nc + c] = g[j];
+  for (int j = 0; j < 4; j++) st2(G, j, nc, c, g[2 * j], g[2 * j + 1]);
   if (F::visc) {
     T *GT = pick<T>(s.gradT, s.gradTf);
     gT[0] = gT[0] / Vt;
     gT[1] = gT[1] / Vt;
-    GT[c] = gT[0];
-    GT[nc + c] = gT[1];
+    st2(GT, 0, nc, c, gT[0], gT[1]);
     double Tc = Ucd[1] / Ucd[0];
     muc = sutherland<double>(k, Tc);
     s.mu[c] = muc;
@@ -326,12 +325,14 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   if (k.order == 1 || frozen_cell(m, c))
     for (int j = 0; j < 4; j++) ph[j] = 0.0;
   T *P = pick<T>(s.phi, s.phif);
-  for (int j = 0; j < 4; j++) P[j * nc + c] = ph[j];
+  st2(P, 0, nc, c, ph[0], ph[1]);
+  st2(P, 1, nc, c, ph[2], ph[3]);
 
   if (F::sst) {
     T *TG = pick<T>(s.tgrad, s.tgradf);
     for (int q = 0; q < 4; q++) tg[q] = tg[q] / Vt;
-    for (int q = 0; q < 4; q++) TG[q * nc + c] = tg[q];
+    st2(TG, 0, nc, c, tg[0], tg[1]);
+    st2(TG, 1, nc, c, tg[2], tg[3]);
     // ComputeSSTf1f2 (SSTKWStrategy.h:695-722)
     T gradVal = dot2(tg[0], tg[1], tg[2], tg[3]);
     T rho = Uc[0], dis = T(m.wdist[c]);
@@ -356,9 +357,12 @@ __device__ __forceinline__ void reconstruct(const DevMesh &m, const DevState &s,
   const T *P = pickc<T>(s.phi, s.phif);
   T v0, v1;
   face_offset<T>(m, f, left, v0, v1);
+  const Pair<T> p01 = ld2(P, 0, nc, cell), p23 = ld2(P, 1, nc, cell);
+  const T ph[4] = {p01.x, p01.y, p23.x, p23.y};
   for (int j = 0; j < 4; j++) {
-    T d2 = dot2(G[(2 * j) * nc + cell], G[(2 * j + 1) * nc + cell], v0, v1);
-    out[j] = d2 * P[j * nc + cell] + Ucell[j];
+    const Pair<T> g = ld2(G, j, nc, cell);
+    T d2 = dot2(g.x, g.y, v0, v1);
+    out[j] = d2 * ph[j] + Ucell[j];
   }
 }
 
@@ -492,31 +496,35 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
     l2r1 = pickc<T>(m.fdx, m.fdxf)[5 * nf + f];
     l2sq = dot2(l2r0, l2r1, l2r0, l2r1);
     for (int j = 0; j < 2; j++) {
-      T a0 = (G[(4 + 2 * j) * nc + L] + G[(4 + 2 * j) * nc + Rr]) * T(0.5);
-      T a1 = (G[(5 + 2 * j) * nc + L] + G[(5 + 2 * j) * nc + Rr]) * T(0.5);
+      const Pair<T> gl = ld2(G, 2 + j, nc, L), gr = ld2(G, 2 + j, nc, Rr);
+      T a0 = (gl.x + gr.x) * T(0.5);
+      T a1 = (gl.y + gr.y) * T(0.5);
       // right - left differences are formed in FP64 (exact for T = double)
       T sc = (T(URd[2 + j] - ULd[2 + j]) - dot2(a0, a1, l2r0, l2r1)) / l2sq;
       gE[j][0] = a0 + l2r0 * sc;
       gE[j][1] = a1 + l2r1 * sc;
     }
-    T a0 = (GT[L] + GT[Rr]) * T(0.5);
-    T a1 = (GT[nc + L] + GT[nc + Rr]) * T(0.5);
+    const Pair<T> tl2 = ld2(GT, 0, nc, L), tr2 = ld2(GT, 0, nc, Rr);
+    T a0 = (tl2.x + tr2.x) * T(0.5);
+    T a1 = (tl2.y + tr2.y) * T(0.5);
     double rightT = URd[1] / URd[0];
     double leftT = ULd[1] / ULd[0];
     T sc = (T(rightT - leftT) - dot2(a0, a1, l2r0, l2r1)) / l2sq;
     gE[2][0] = a0 + l2r0 * sc;
     gE[2][1] = a1 + l2r1 * sc;
   } else {
-    gE[0][0] = G[4 * nc + L];
-    gE[0][1] = G[5 * nc + L];
-    gE[1][0] = G[6 * nc + L];
-    gE[1][1] = G[7 * nc + L];
+    const Pair<T> gu = ld2(G, 2, nc, L), gv = ld2(G, 3, nc, L);
+    gE[0][0] = gu.x;
+    gE[0][1] = gu.y;
+    gE[1][0] = gv.x;
+    gE[1][1] = gv.y;
     if (Rr == kWallD) {
       gE[2][0] = 0;
       gE[2][1] = 0;
     } else {
-      gE[2][0] = GT[L];
-      gE[2][1] = GT[nc + L];
+      const Pair<T> gt = ld2(GT, 0, nc, L);
+      gE[2][0] = gt.x;
+      gE[2][1] = gt.y;
     }
   }
   T taoxx = efficient * (T(2.0) * gE[0][0] - gE[1][1]);
@@ -570,8 +578,9 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   if (inner) {
     face_f1 = T(0.5) * (F1[L] + F1[Rr]);
     for (int q = 0; q < 2; q++) {
-      gq[q][0] = (TG[(2 * q) * nc + L] + TG[(2 * q) * nc + Rr]) * T(0.5);
-      gq[q][1] = (TG[(2 * q + 1) * nc + L] + TG[(2 * q + 1) * nc + Rr]) * T(0.5);
+      const Pair<T> ql = ld2(TG, q, nc, L), qr = ld2(TG, q, nc, Rr);
+      gq[q][0] = (ql.x + qr.x) * T(0.5);
+      gq[q][1] = (ql.y + qr.y) * T(0.5);
     }
     for (int q = 0; q < 2; q++) {
       const T dq = T(s.tU[q * nc + Rr] - s.tU[q * nc + L]);
@@ -581,8 +590,9 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   } else {
     face_f1 = F1[L];
     for (int q = 0; q < 2; q++) {
-      gq[q][0] = TG[(2 * q) * nc + L];
-      gq[q][1] = TG[(2 * q + 1) * nc + L];
+      const Pair<T> ql = ld2(TG, q, nc, L);
+      gq[q][0] = ql.x;
+      gq[q][1] = ql.y;
     }
   }
   T sigma_k = face_f1 * T(SST_SK1) + (1 - face_f1) * T(SST_SK2);
@@ -777,8 +787,8 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
       tW[q] = s.tW[q * nc + c];
     }
     const T *G = pickc<T>(s.grad, s.gradf);
-    g5 = double(G[5 * nc + c]);
-    g6 = double(G[6 * nc + c]);
+    g5 = double(G[pidx(5, nc, c)]);
+    g6 = double(G[pidx(6, nc, c)]);
     f2 = double(pickc<T>(s.F2, s.F2f)[c]);
   }
   double dW[4], Wn[4], Un[4];
@@ -895,11 +905,11 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
     double tUc[2] = {s.tU[c], s.tU[nc + c]};
     const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
     double tgc[4];
-    for (int q = 0; q < 4; q++) tgc[q] = double(TG[q * nc + c]);
+    for (int q = 0; q < 4; q++) tgc[q] = double(TG[pidx(q, nc, c)]);
     const double F1c = double(pickc<T>(s.F1, s.F1f)[c]), mutc = s.mut[c];
     const double rho = s.U[c];
-    double gux = double(G[4 * nc + c]), guy = double(G[5 * nc + c]);
-    double gvx = double(G[6 * nc + c]), gvy = double(G[7 * nc + c]);
+    double gux = double(G[pidx(4, nc, c)]), guy = double(G[pidx(5, nc, c)]);
+    double gvx = double(G[pidx(6, nc, c)]), gvy = double(G[pidx(7, nc, c)]);
     double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
     double ddls1 = (guy + gvx) * (guy + gvx);
     double ddls2 = 2.0 / 3.0 * rho * tUc[0] * (gux + gvy);
@@ -2045,6 +2055,20 @@ static int clear_case_status(usg_ctx *c) {
   return USG_OK;
 }
 
+// A paired device array ([k/2][nc][2], pidx) as [nc][nv] (FP64 or FP32).
+template <typename T>
+static int pairs_to_aos(usg_ctx *c, const T *dev, int nv, double *out, long count, bool has) {
+  const int nc = c->hm.nc;
+  if (count != (long)nv * nc) return fail(USG_ERR_ARG, "count mismatch");
+  if (!has || !dev) return fail(USG_ERR_ARG, "field not available for this flow type");
+  std::vector<T> tmp((size_t)nv * nc);
+  HIPCHK(hipMemcpyAsync(tmp.data(), dev, tmp.size() * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < nc; i++)
+    for (int k = 0; k < nv; k++) out[(size_t)i * nv + k] = (double)tmp[pidx(k, nc, i)];
+  return USG_OK;
+}
+
 static void drop_graphs(usg_ctx *c);
 
 extern "C" {
@@ -2675,15 +2699,16 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
     case USG_F_CONS: return soa_to_aos(c, s.W, 4, out, count, true);
     case USG_F_RES: return soa_to_aos(c, s.R, 4, out, count, true);
     case USG_F_GRAD:
-      return s.gradf ? soa_to_aos(c, s.gradf, 8, out, count, true) : soa_to_aos(c, s.grad, 8, out, count, true);
+      return s.gradf ? pairs_to_aos(c, s.gradf, 8, out, count, true) : pairs_to_aos(c, s.grad, 8, out, count, true);
     case USG_F_LAMC: return soa_to_aos(c, s.lamc, 1, out, count, true);
     case USG_F_PHI:
-      return s.phif ? soa_to_aos(c, s.phif, 4, out, count, true) : soa_to_aos(c, s.phi, 4, out, count, true);
+      return s.phif ? pairs_to_aos(c, s.phif, 4, out, count, true) : pairs_to_aos(c, s.phi, 4, out, count, true);
     case USG_F_DW1: return soa_to_aos(c, s.dW1, 4, out, count, true);
     case USG_F_DW: return soa_to_aos(c, s.dW, 4, out, count, true);
     case USG_F_LAMV: return soa_to_aos(c, s.lamv, 1, out, count, visc);
     case USG_F_GRADT:
-      return s.gradTf ? soa_to_aos(c, s.gradTf, 2, out, count, visc) : soa_to_aos(c, s.gradT, 2, out, count, visc);
+      return s.gradTf ? pairs_to_aos(c, s.gradTf, 2, out, count, visc)
+                      : pairs_to_aos(c, s.gradT, 2, out, count, visc);
     case USG_F_TURB: return soa_to_aos(c, s.tU, 2, out, count, sst);
     case USG_F_TRES: return soa_to_aos(c, s.tR, 2, out, count, sst);
     case USG_F_MUT: return soa_to_aos(c, s.mut, 1, out, count, sst);
