@@ -86,18 +86,18 @@ __device__ __forceinline__ bool frozen_cell(const DevMesh &m, int c) { return m.
 
 // State, structure-of-arrays ([var*nc + cell]).
 struct DevState {
-  double *U, *W;        // current primitive / conservative
-  double *Un, *Wn;      // next iterate (double-buffered update)
+  double *U, *W;        // current primitive / conservative, [2][nc][2] (pidx)
+  double *Un, *Wn;      // next iterate (double-buffered update), [2][nc][2]
   double *grad;         // [4][nc][2]: (d(var)/dx, d(var)/dy) of var v in pair v (pidx(2v+d))
   double *gradT;        // [1][nc][2]
   double *lamc, *lamv;  // [nc]
   double *mu;           // [nc] laminar viscosity at the cell
   double *phi;          // [2][nc][2] (pidx(var))
   double *R;            // [4][nc]
-  double *dW1, *dW;     // [4][nc]
+  double *dW1, *dW;     // [2][nc][2] (pidx)
   double *dt;           // [nc] LU-SGS local time step
   double *dt_rk;        // [nc] RK5 time step, frozen at stage 0
-  double *W0;           // [4][nc] RK5 stage origin
+  double *W0;           // [2][nc][2] RK5 stage origin (pidx)
   double *FF;           // face terms (q: flux 0-3, spectral 4, viscous flux 5-8, viscous
                         // spectral 9, SST 10-11), oriented out of the left cell, in
                         // pairs [pair][nf][2] (rec_pair / rec_comp in the solver)
@@ -106,13 +106,14 @@ struct DevState {
   double *rA;           // [4][nc] LU-SGS r_A of each interior slot of a cell (from the
                         // neighbour across it), written by the face kernel
   // SST
-  double *tU, *tW, *tUn, *tWn;  // [2][nc]
+  double *tU, *tW, *tUn, *tWn;  // [1][nc][2]: (k, omega) side by side (pidx)
   double *tgrad;                // [2][nc][2]: (d/dx, d/dy) of k, omega (pidx(2q+d))
   double *F1, *F2, *mut, *mutn; // [nc]
-  double *tR, *tdW1, *tdW;      // [2][nc]
+  double *tR;                   // [2][nc]
+  double *tdW1, *tdW;           // [1][nc][2] (pidx)
   int *err;                     // [2] invariant violation counters
   // dual time stepping: the conservative state at the two previous
-  // physical time levels, W^n and W^(n-1) ([4][nc]; null when off)
+  // physical time levels, W^n and W^(n-1) ([2][nc][2]; null when off)
   double *Wt1, *Wt2;
   // face states of the boundary faces for the gradient kernel, [4][nbf]
   // (k_bstate at the start of every residual)
@@ -157,6 +158,21 @@ __device__ __forceinline__ void st2(T *a, int p, int nc, int c, T x, T y) {
 }
 __host__ __device__ __forceinline__ size_t pidx(int k, int nc, int c) {
   return ((size_t)(k >> 1) * nc + c) * 2 + (k & 1);
+}
+// The state arrays that neighbours gather -- U, W and their second buffers,
+// the sweeps' increments dW1, dW, the SST state tU, tW, tdW1, tdW, and W0,
+// Wt1, Wt2 -- use the same pairs: (rho, p), (u, v) of U; (rho, rhoE),
+// (rhou, rhov) of W; (k, omega) of tU.
+__device__ __forceinline__ void load4p(const double *a, int nc, int c, double *v) {
+  const double2 x = ld2(a, 0, nc, c), y = ld2(a, 1, nc, c);
+  v[0] = x.x;
+  v[1] = x.y;
+  v[2] = y.x;
+  v[3] = y.y;
+}
+__device__ __forceinline__ void store4p(double *a, int nc, int c, const double *v) {
+  st2(a, 0, nc, c, v[0], v[1]);
+  st2(a, 1, nc, c, v[2], v[3]);
 }
 
 struct Consts {

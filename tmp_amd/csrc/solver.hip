@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_bstate(const DevMesh m, const DevSta
   const int f = m.bface[b], c = m.flc[f], nc = m.nc;
   const Consts k = case_consts<B>(m, k_, c);
   double Ul[4], Ue[4];
-  load4(s.U, nc, c, Ul);
+  load4p(s.U, nc, c, Ul);
   if (!boundary_state<VISC>(k, m.frc[f], Ul, m.fnx[f], m.fny[f], Ue)) {
     atomicAdd(&s.err[1], 1);
     if constexpr (B) atomicOr(&s.err[2 + m.ccase[c]], 2);
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   const int nc = m.nc;
   const T *fnx = fnx_of<T>(m), *fny = fny_of<T>(m), *fA = fA_of<T>(m);
   double Ucd[4];
-  load4(s.U, nc, c, Ucd);
+  load4p(s.U, nc, c, Ucd);
   T Uc[4];
   for (int j = 0; j < 4; j++) Uc[j] = T(Ucd[j]);
   const double V = m.vol[c];
@@ -157,8 +157,8 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
   T tUc[2] = {0, 0};
   double muc = 0;
   if (F::sst) {
-    tUc[0] = T(s.tU[c]);
-    tUc[1] = T(s.tU[nc + c]);
+    tUc[0] = T(s.tU[pidx(0, nc, c)]);
+    tUc[1] = T(s.tU[pidx(1, nc, c)]);
   }
   // FP32 (mixed) path: face values enter Green-Gauss as differences from the
   // cell value, formed in FP64 -- sum (phi_f - phi_c) n A equals
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
     double Ued[4];
     if (o.nb >= 0) {
       double Unbd[4];
-      load4(s.U, nc, o.nb, Unbd);
+      load4p(s.U, nc, o.nb, Unbd);
       for (int j = 0; j < 4; j++) {
         if constexpr (kDiff) {
           Ued[j] = (Unbd[j] + Ucd[j]) / 2;
@@ -232,11 +232,11 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
         // ComputeFaceTValue (SSTKWStrategy.h:616-659)
         T te[2];
         if constexpr (kDiff) {
-          const double tc0 = s.tU[c], tc1 = s.tU[nc + c];
+          const double tc0 = s.tU[pidx(0, nc, c)], tc1 = s.tU[pidx(1, nc, c)];
           double ted[2];
           if (o.nb >= 0) {
-            ted[0] = 0.5 * (s.tU[o.nb] - tc0);
-            ted[1] = 0.5 * (s.tU[nc + o.nb] - tc1);
+            ted[0] = 0.5 * (s.tU[pidx(0, nc, o.nb)] - tc0);
+            ted[1] = 0.5 * (s.tU[pidx(1, nc, o.nb)] - tc1);
           } else if (o.nb == kWallD) {
             const double dis = m.wdist[c];
             ted[0] = 0.0 - tc0;
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_grad(const DevMesh m, const DevState
           te[1] = T(ted[1]);
         } else {
           if (o.nb >= 0) {
-            T t0 = T(s.tU[o.nb]), t1 = T(s.tU[nc + o.nb]);
+            T t0 = T(s.tU[pidx(0, nc, o.nb)]), t1 = T(s.tU[pidx(1, nc, o.nb)]);
             te[0] = o.left ? T(0.5) * (tUc[0] + t0) : T(0.5) * (t0 + tUc[0]);
             te[1] = o.left ? T(0.5) * (tUc[1] + t1) : T(0.5) * (t1 + tUc[1]);
           } else if (o.nb == kWallD) {
@@ -422,12 +422,12 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   const Consts k = case_consts<B>(m, k_, L);
   const T n0 = fnx_of<T>(m)[f], n1 = fny_of<T>(m)[f], A = fA_of<T>(m)[f];
   double ULd[4], URd[4] = {0, 0, 0, 0};
-  load4(s.U, nc, L, ULd);
+  load4p(s.U, nc, L, ULd);
   T UL[4], UR[4], Ue[4], Fx[4];
   for (int j = 0; j < 4; j++) UL[j] = T(ULd[j]);
   const bool inner = Rr >= 0;
   if (inner) {
-    load4(s.U, nc, Rr, URd);
+    load4p(s.U, nc, Rr, URd);
     for (int j = 0; j < 4; j++) UR[j] = T(URd[j]);
     T Ua[4], Ub[4];
     reconstruct<T>(m, s, L, f, true, UL, Ua);
@@ -544,10 +544,10 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
   if (!F::sst) return;
   // face turbulence values (ComputeFaceTValue, SSTKWStrategy.h:616-659)
   const T Vn = dot2(Ue[2], Ue[3], n0, n1);
-  T tl[2] = {T(s.tU[L]), T(s.tU[nc + L])}, tr[2] = {0, 0}, te[2];
+  T tl[2] = {T(s.tU[pidx(0, nc, L)]), T(s.tU[pidx(1, nc, L)])}, tr[2] = {0, 0}, te[2];
   if (inner) {
-    tr[0] = T(s.tU[Rr]);
-    tr[1] = T(s.tU[nc + Rr]);
+    tr[0] = T(s.tU[pidx(0, nc, Rr)]);
+    tr[1] = T(s.tU[pidx(1, nc, Rr)]);
     te[0] = T(0.5) * (tl[0] + tr[0]);
     te[1] = T(0.5) * (tl[1] + tr[1]);
   } else if (Rr == kWallD) {
@@ -583,7 +583,7 @@ __device__ __forceinline__ void face_terms(const DevMesh &m, const DevState &s, 
       gq[q][1] = (ql.y + qr.y) * T(0.5);
     }
     for (int q = 0; q < 2; q++) {
-      const T dq = T(s.tU[q * nc + Rr] - s.tU[q * nc + L]);
+      const T dq = T(s.tU[pidx(q, nc, Rr)] - s.tU[pidx(q, nc, L)]);
       gq[q][0] = gq[q][0] + (dq - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r0 / l2sq;
       gq[q][1] = gq[q][1] + (dq - (gq[q][0] * l2r0 + gq[q][1] * l2r1)) * l2r1 / l2sq;
     }
@@ -709,10 +709,10 @@ __device__ __forceinline__ void fwd_store_with(const DevMesh &m, const DevState 
   for (int j = 0; j < 4; j++) dW1[j] = fz ? 0.0 : (-R[j] - dFi[j]) / diag;
   if (F::sst)
     for (int q = 0; q < 2; q++) tdW1[q] = fz ? 0.0 : (-tR[q] - tFi[q]) / tD[q];
-  store4(s.dW1, nc, c, dW1);
+  store4p(s.dW1, nc, c, dW1);
   if (F::sst) {
-    s.tdW1[c] = tdW1[0];
-    s.tdW1[nc + c] = tdW1[1];
+    s.tdW1[pidx(0, nc, c)] = tdW1[0];
+    s.tdW1[pidx(1, nc, c)] = tdW1[1];
   }
 }
 
@@ -738,24 +738,24 @@ __device__ __forceinline__ void carry_state(const DevMesh &m, const DevState &s,
   using F = Flow<FLOW>;
   const int nc = m.nc;
   double W[4], U[4], tw[2] = {0.0, 0.0}, tu[2] = {0.0, 0.0}, mt = 0.0;
-  load4(s.W, nc, c, W);
-  load4(s.U, nc, c, U);
+  load4p(s.W, nc, c, W);
+  load4p(s.U, nc, c, U);
   if (F::sst) {
     for (int q = 0; q < 2; q++) {
-      tw[q] = s.tW[q * nc + c];
-      tu[q] = s.tU[q * nc + c];
+      tw[q] = s.tW[pidx(q, nc, c)];
+      tu[q] = s.tU[pidx(q, nc, c)];
     }
     mt = s.mut[c];
   }
   const double zero[4] = {0.0, 0.0, 0.0, 0.0};
-  store4(s.dW, nc, c, zero);
-  store4(s.Wn, nc, c, W);
-  store4(s.Un, nc, c, U);
+  store4p(s.dW, nc, c, zero);
+  store4p(s.Wn, nc, c, W);
+  store4p(s.Un, nc, c, U);
   if (F::sst) {
     for (int q = 0; q < 2; q++) {
-      s.tdW[q * nc + c] = 0.0;
-      s.tWn[q * nc + c] = tw[q];
-      s.tUn[q * nc + c] = tu[q];
+      s.tdW[pidx(q, nc, c)] = 0.0;
+      s.tWn[pidx(q, nc, c)] = tw[q];
+      s.tUn[pidx(q, nc, c)] = tu[q];
     }
     s.mutn[c] = mt;
   }
@@ -780,11 +780,11 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
     return;
   }
   double W[4], tD[2] = {1.0, 1.0}, tW[2] = {0.0, 0.0}, g5 = 0.0, g6 = 0.0, f2 = 0.0;
-  load4(s.W, nc, c, W);
+  load4p(s.W, nc, c, W);
   if (F::sst) {
     for (int q = 0; q < 2; q++) {
       tD[q] = s.tD[q * nc + c];
-      tW[q] = s.tW[q * nc + c];
+      tW[q] = s.tW[pidx(q, nc, c)];
     }
     const T *G = pickc<T>(s.grad, s.gradf);
     g5 = double(G[pidx(5, nc, c)]);
@@ -797,14 +797,14 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
     Wn[j] = W[j] + dW[j];
   }
   prim_of(Wn, Un);
-  store4(s.dW, nc, c, dW);
-  store4(s.Wn, nc, c, Wn);
-  store4(s.Un, nc, c, Un);
+  store4p(s.dW, nc, c, dW);
+  store4p(s.Wn, nc, c, Wn);
+  store4p(s.Un, nc, c, Un);
   if (F::sst) {
     double tdw[2], tw[2];
     for (int q = 0; q < 2; q++) {
       tdw[q] = tdW1[q] - tFi[q] / tD[q];
-      s.tdW[q * nc + c] = tdw[q];
+      s.tdW[pidx(q, nc, c)] = tdw[q];
       tw[q] = tW[q] + tdw[q];
     }
     double rho = Un[0];
@@ -818,10 +818,10 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
         tw[q] = rho * fl;
       }
     }
-    s.tWn[c] = tw[0];
-    s.tWn[nc + c] = tw[1];
-    s.tUn[c] = tu[0];
-    s.tUn[nc + c] = tu[1];
+    s.tWn[pidx(0, nc, c)] = tw[0];
+    s.tWn[pidx(1, nc, c)] = tw[1];
+    s.tUn[pidx(0, nc, c)] = tu[0];
+    s.tUn[pidx(1, nc, c)] = tu[1];
     double curl = fabs(g6 - g5);
     double h = smax(SST_A1 * tw[1] / rho, f2 * curl);
     s.mutn[c] = SST_A1 * tw[0] / h;
@@ -881,7 +881,7 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
     // dual time stepping: the physical-time derivative of W by backward
     // differences, added after the face fluxes
     for (int j = 0; j < 4; j++)
-      Rc[j] += V * (k.dts[0] * s.W[j * nc + c] + k.dts[1] * s.Wt1[j * nc + c] + k.dts[2] * s.Wt2[j * nc + c]);
+      Rc[j] += V * (k.dts[0] * s.W[pidx(j, nc, c)] + k.dts[1] * s.Wt1[pidx(j, nc, c)] + k.dts[2] * s.Wt2[pidx(j, nc, c)]);
   }
   double lamc = 0, lamv = 0;
   for (int i = 0; i < ns; i++) {
@@ -902,12 +902,12 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
   // the loads would hold the later loads behind it)
   double trc[2] = {0.0, 0.0}, tdiag[2] = {0.0, 0.0};
   if (F::sst) {
-    double tUc[2] = {s.tU[c], s.tU[nc + c]};
+    double tUc[2] = {s.tU[pidx(0, nc, c)], s.tU[pidx(1, nc, c)]};
     const T *G = pickc<T>(s.grad, s.gradf), *TG = pickc<T>(s.tgrad, s.tgradf);
     double tgc[4];
     for (int q = 0; q < 4; q++) tgc[q] = double(TG[pidx(q, nc, c)]);
     const double F1c = double(pickc<T>(s.F1, s.F1f)[c]), mutc = s.mut[c];
-    const double rho = s.U[c];
+    const double rho = s.U[pidx(0, nc, c)];
     double gux = double(G[pidx(4, nc, c)]), guy = double(G[pidx(5, nc, c)]);
     double gvx = double(G[pidx(6, nc, c)]), gvy = double(G[pidx(7, nc, c)]);
     double ddls0 = 2.0 / 3.0 * ((gux - gvy) * (gux - gvy));
@@ -1078,7 +1078,7 @@ __device__ __forceinline__ void df_term(const DevMesh &m, const DevState &s, int
   const double n0 = m.snx[sc], n1 = m.sny[sc], A = m.sA[sc];
   double W[4], Wp[4], Fc[4], Fcp[4];
   for (int j = 0; j < 4; j++) {
-    const double W0 = s.W[j * nc + nb];
+    const double W0 = s.W[pidx(j, nc, nb)];
     Wp[j] = W0 + dW[j];
     W[j] = W0;
   }
@@ -1096,7 +1096,7 @@ __device__ __forceinline__ void df_term(const DevMesh &m, const DevState &s, int
     double up = Wp[2] / Wp[0], vp = Wp[3] / Wp[0];
     double vnp = dot2(up, vp, n0, n1);
     for (int q = 0; q < 2; q++) {
-      const double tw = s.tW[q * nc + nb];
+      const double tw = s.tW[pidx(q, nc, nb)];
       double twp = tw + tdw[q];
       double fc = tw * vn, fcp = twp * vnp;
       double DFc = left ? (fcp - fc) : (fc - fcp);
@@ -1121,9 +1121,9 @@ __device__ __forceinline__ void df_sum(const DevMesh &m, const DevState &s, int 
     if (o.nb < 0) continue;
     if (FWD ? !(o.nb < c) : !(o.nb > c)) continue;
     double dW[4], tdw[2] = {0.0, 0.0}, t[4], tt[2] = {0.0, 0.0};
-    for (int j = 0; j < 4; j++) dW[j] = dsrc[j * nc + o.nb];
+    for (int j = 0; j < 4; j++) dW[j] = dsrc[pidx(j, nc, o.nb)];
     if (F::sst)
-      for (int q = 0; q < 2; q++) tdw[q] = tsrc[q * nc + o.nb];
+      for (int q = 0; q < 2; q++) tdw[q] = tsrc[pidx(q, nc, o.nb)];
     df_term<FLOW>(m, s, c, sl, o.left, o.nb, dW, tdw, t, tt);
     for (int j = 0; j < 4; j++) dFi[j] += t[j];
     if (F::sst)
@@ -1154,9 +1154,9 @@ __device__ __forceinline__ void increment_of(const DevMesh &m, const DevState &s
     if (F::sst)
       for (int q = 0; q < 2; q++) tdw[q] = (-s.tR[q * nc + u] - tFu[q]) / s.tD[q * nc + u];
   } else {
-    for (int j = 0; j < 4; j++) dW[j] = s.dW1[j * nc + u] - dFu[j] / diag;
+    for (int j = 0; j < 4; j++) dW[j] = s.dW1[pidx(j, nc, u)] - dFu[j] / diag;
     if (F::sst)
-      for (int q = 0; q < 2; q++) tdw[q] = s.tdW1[q * nc + u] - tFu[q] / s.tD[q * nc + u];
+      for (int q = 0; q < 2; q++) tdw[q] = s.tdW1[pidx(q, nc, u)] - tFu[q] / s.tD[q * nc + u];
   }
 }
 
@@ -1199,9 +1199,9 @@ __device__ __forceinline__ void sweep_tail_cell(const DevMesh &m, const DevState
     } else {
       const double *dsrc = FWD ? s.dW1 : s.dW;
       const double *tsrc = FWD ? s.tdW1 : s.tdW;
-      for (int j = 0; j < 4; j++) dW[j] = dsrc[j * nc + o.nb];
+      for (int j = 0; j < 4; j++) dW[j] = dsrc[pidx(j, nc, o.nb)];
       if (F::sst)
-        for (int r = 0; r < 2; r++) tdw[r] = tsrc[r * nc + o.nb];
+        for (int r = 0; r < 2; r++) tdw[r] = tsrc[pidx(r, nc, o.nb)];
     }
     df_term<FLOW>(m, s, c, q, o.left, o.nb, dW, tdw, t, tt);
   }
@@ -1228,9 +1228,9 @@ __device__ __forceinline__ void sweep_tail_cell(const DevMesh &m, const DevState
     if (last) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   } else {
     double dW1[4], tdW1[2] = {0.0, 0.0};
-    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
+    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[pidx(j, nc, c)];
     if (F::sst)
-      for (int r = 0; r < 2; r++) tdW1[r] = s.tdW1[r * nc + c];
+      for (int r = 0; r < 2; r++) tdW1[r] = s.tdW1[pidx(r, nc, c)];
     bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, dFi, tFi, diag);
   }
 }
@@ -1270,9 +1270,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) voi
     if (last) bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, zero, zero, diag);
   } else {
     double dW1[4], tdW1[2] = {0.0, 0.0};
-    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[j * nc + c];
+    for (int j = 0; j < 4; j++) dW1[j] = s.dW1[pidx(j, nc, c)];
     if (F::sst)
-      for (int q = 0; q < 2; q++) tdW1[q] = s.tdW1[q * nc + c];
+      for (int q = 0; q < 2; q++) tdW1[q] = s.tdW1[pidx(q, nc, c)];
     bwd_store<FLOW, T, B>(m, s, k, c, dW1, tdW1, dFi, tFi, diag);
   }
 }
@@ -1290,10 +1290,10 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(const DevMesh m, const DevS
   }
   const double dtc = s.dt_rk[c];
   double Wn[4], Un[4];
-  for (int j = 0; j < 4; j++) Wn[j] = s.W0[j * nc + c] + alpha * dtc * (-s.R[j * nc + c]) / V;
+  for (int j = 0; j < 4; j++) Wn[j] = s.W0[pidx(j, nc, c)] + alpha * dtc * (-s.R[j * nc + c]) / V;
   prim_of(Wn, Un);
-  store4(s.W, nc, c, Wn);
-  store4(s.U, nc, c, Un);
+  store4p(s.W, nc, c, Wn);
+  store4p(s.U, nc, c, Un);
 }
 
 // initializeConsData of the current U into W (and W0)
@@ -1302,9 +1302,9 @@ __global__ __launch_bounds__(kBlock) void k_cons(const DevMesh m, const DevState
   const int nc = m.nc;
   if (c >= nc) return;
   double U[4], W[4];
-  load4(s.U, nc, c, U);
+  load4p(s.U, nc, c, U);
   cons_of(U, W);
-  store4(Wdst, nc, c, W);
+  store4p(Wdst, nc, c, W);
 }
 
 // SSTkomegaModel::initializeConsData (SSTKWStrategy.h:97-109)
@@ -1312,9 +1312,9 @@ __global__ __launch_bounds__(kBlock) void k_tcons(const DevMesh m, const DevStat
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   const int nc = m.nc;
   if (c >= nc) return;
-  double rho = s.U[c];
-  s.tW[c] = s.tU[c] * rho;
-  s.tW[nc + c] = s.tU[nc + c] * rho;
+  double rho = s.U[pidx(0, nc, c)];
+  s.tW[pidx(0, nc, c)] = s.tU[pidx(0, nc, c)] * rho;
+  s.tW[pidx(1, nc, c)] = s.tU[pidx(1, nc, c)] * rho;
 }
 
 // Free-stream cold start (Euler2D.cpp:153-171, SSTKWStrategy.h:582-613)
@@ -1323,10 +1323,10 @@ __global__ __launch_bounds__(kBlock) void k_freestream(const DevMesh m, const De
   const int nc = m.nc;
   if (c >= nc) return;
   const Consts k = m.kc ? m.kc[m.ccase[c]] : k_;
-  for (int j = 0; j < 4; j++) s.U[j * nc + c] = k.fs[j];
+  for (int j = 0; j < 4; j++) s.U[pidx(j, nc, c)] = k.fs[j];
   if (sst) {
-    s.tU[c] = k.fsnd_turb[0];
-    s.tU[nc + c] = k.fsnd_turb[1];
+    s.tU[pidx(0, nc, c)] = k.fsnd_turb[0];
+    s.tU[pidx(1, nc, c)] = k.fsnd_turb[1];
     s.mut[c] = k.fsnd_mut;
   }
 }
@@ -1409,7 +1409,7 @@ __device__ void wall_shear(const DevMesh &m, const DevState &s, const Consts &k,
                            double &gy) {
   const int nc = m.nc;
   double Ucd[4];
-  load4(s.U, nc, c, Ucd);
+  load4p(s.U, nc, c, Ucd);
   double g[4] = {0, 0, 0, 0};  // du/dx, du/dy, dv/dx, dv/dy
   for (int sl = 0; sl < 4; sl++) {
     Slot o;
@@ -1418,7 +1418,7 @@ __device__ void wall_shear(const DevMesh &m, const DevState &s, const Consts &k,
     double Ue[4];
     if (o.nb >= 0) {
       double Unbd[4];
-      load4(s.U, nc, o.nb, Unbd);
+      load4p(s.U, nc, o.nb, Unbd);
       for (int j = 2; j < 4; j++) Ue[j] = o.left ? (Ucd[j] + Unbd[j]) / 2 : (Unbd[j] + Ucd[j]) / 2;
     } else {
       boundary_state<true>(k, o.nb, Ucd, m.fnx[o.f], m.fny[o.f], Ue);
@@ -1459,7 +1459,7 @@ __global__ __launch_bounds__(kBlock) void k_forces(const DevMesh m, const DevSta
   const int i0 = woff[blockIdx.x], nw = woff[blockIdx.x + 1] - i0;
   for (int i = threadIdx.x; i < nw; i += kBlock) {
     int f = wall_faces[i0 + i], c = wall_cells[i0 + i];
-    double p = s.U[m.nc + c];
+    double p = s.U[pidx(1, m.nc, c)];
     double gx = p * m.fnx[f] * m.fA[f], gy = p * m.fny[f] * m.fA[f];
     if (VISC) {
       const Consts k = m.kc ? m.kc[m.ccase[c]] : k_;
@@ -1541,7 +1541,7 @@ __global__ __launch_bounds__(kBlock) void k_overset(const DevMesh m, const DevSt
       }
 #pragma unroll
     for (int k = 0; k < kChunk; k++)
-      if (i0 + k < i1) x[k] = s.U[j * nc + d[k]];
+      if (i0 + k < i1) x[k] = s.U[pidx(j, nc, d[k])];
 #pragma unroll
     for (int k = 0; k < kChunk; k++)
       if (i0 + k < i1) u += wk[k] * x[k];
@@ -1549,8 +1549,8 @@ __global__ __launch_bounds__(kBlock) void k_overset(const DevMesh m, const DevSt
   double U[4], W[4];
   for (int q = 0; q < 4; q++) U[q] = __shfl(u, q, 4);
   cons_of(U, W);
-  s.U[j * nc + c] = U[j];
-  s.W[j * nc + c] = W[j];
+  s.U[pidx(j, nc, c)] = U[j];
+  s.W[pidx(j, nc, c)] = W[j];
 }
 
 // Face states of the current U (Euler2D::updateEdgeValues through
@@ -1565,10 +1565,10 @@ __global__ __launch_bounds__(kBlock) void k_uedge(const DevMesh m, const DevStat
   const int L = m.flc[f], R = m.frc[f];
   const Consts k = m.kc ? m.kc[m.ccase[L]] : k_;
   double UL[4], Ue[4];
-  load4(s.U, nc, L, UL);
+  load4p(s.U, nc, L, UL);
   if (R >= 0) {
     double UR[4];
-    load4(s.U, nc, R, UR);
+    load4p(s.U, nc, R, UR);
     for (int j = 0; j < 4; j++) Ue[j] = (UL[j] + UR[j]) / 2;
   } else {
     boundary_state<VISC>(k, R, UL, m.fnx[f], m.fny[f], Ue);
@@ -2453,13 +2453,13 @@ int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
   const int nc = c->hm.nc;
   std::vector<double> soa(4 * (size_t)nc);
   for (int i = 0; i < nc; i++)
-    for (int j = 0; j < 4; j++) soa[(size_t)j * nc + i] = U[4 * (size_t)i + j];
+    for (int j = 0; j < 4; j++) soa[pidx(j, nc, i)] = U[4 * (size_t)i + j];
   HIPCHK(hipMemcpyAsync(c->ds.U, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
   if (c->prm.flow == USG_SST && turb) {
     std::vector<double> t(2 * (size_t)nc);
     for (int i = 0; i < nc; i++)
-      for (int j = 0; j < 2; j++) t[(size_t)j * nc + i] = turb[2 * (size_t)i + j];
+      for (int j = 0; j < 2; j++) t[pidx(j, nc, i)] = turb[2 * (size_t)i + j];
     HIPCHK(hipMemcpyAsync(c->ds.tU, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
   }
@@ -2469,11 +2469,14 @@ int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
   return USG_OK;
 }
 
-static int upload_aos(usg_ctx *c, const double *src, int width, double *dst) {
+// [nc][width] host values into a device array, paired ([width/2][nc][2],
+// pidx) or one array per component.
+static int upload_aos(usg_ctx *c, const double *src, int width, double *dst, bool paired) {
   const int nc = c->hm.nc;
   std::vector<double> soa((size_t)width * nc);
   for (int i = 0; i < nc; i++)
-    for (int j = 0; j < width; j++) soa[(size_t)j * nc + i] = src[(size_t)width * i + j];
+    for (int j = 0; j < width; j++)
+      soa[paired ? pidx(j, nc, i) : (size_t)j * nc + i] = src[(size_t)width * i + j];
   HIPCHK(hipMemcpyAsync(dst, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));  // soa is freed on return
   return USG_OK;
@@ -2486,20 +2489,20 @@ int usg_set_state(usg_ctx *c, const double *U, const double *W, const double *tu
   if (sst && (!turb || !mut)) return fail(USG_ERR_ARG, "SST state needs turb and mut");
   int rc = clear_case_status(c);
   if (rc) return rc;
-  if ((rc = upload_aos(c, U, 4, c->ds.U))) return rc;
+  if ((rc = upload_aos(c, U, 4, c->ds.U, true))) return rc;
   if (W) {
-    if ((rc = upload_aos(c, W, 4, c->ds.W))) return rc;
+    if ((rc = upload_aos(c, W, 4, c->ds.W, true))) return rc;
   } else {
     hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
   }
   if (sst) {
-    if ((rc = upload_aos(c, turb, 2, c->ds.tU))) return rc;
+    if ((rc = upload_aos(c, turb, 2, c->ds.tU, true))) return rc;
     if (tcons) {
-      if ((rc = upload_aos(c, tcons, 2, c->ds.tW))) return rc;
+      if ((rc = upload_aos(c, tcons, 2, c->ds.tW, true))) return rc;
     } else {
       hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
     }
-    if ((rc = upload_aos(c, mut, 1, c->ds.mut))) return rc;
+    if ((rc = upload_aos(c, mut, 1, c->ds.mut, false))) return rc;
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2695,28 +2698,28 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
   const DevState &s = c->ds;
   const bool visc = c->prm.flow != USG_EULER, sst = c->prm.flow == USG_SST;
   switch (field) {
-    case USG_F_PRIM: return soa_to_aos(c, s.U, 4, out, count, true);
-    case USG_F_CONS: return soa_to_aos(c, s.W, 4, out, count, true);
+    case USG_F_PRIM: return pairs_to_aos(c, s.U, 4, out, count, true);
+    case USG_F_CONS: return pairs_to_aos(c, s.W, 4, out, count, true);
     case USG_F_RES: return soa_to_aos(c, s.R, 4, out, count, true);
     case USG_F_GRAD:
       return s.gradf ? pairs_to_aos(c, s.gradf, 8, out, count, true) : pairs_to_aos(c, s.grad, 8, out, count, true);
     case USG_F_LAMC: return soa_to_aos(c, s.lamc, 1, out, count, true);
     case USG_F_PHI:
       return s.phif ? pairs_to_aos(c, s.phif, 4, out, count, true) : pairs_to_aos(c, s.phi, 4, out, count, true);
-    case USG_F_DW1: return soa_to_aos(c, s.dW1, 4, out, count, true);
-    case USG_F_DW: return soa_to_aos(c, s.dW, 4, out, count, true);
+    case USG_F_DW1: return pairs_to_aos(c, s.dW1, 4, out, count, true);
+    case USG_F_DW: return pairs_to_aos(c, s.dW, 4, out, count, true);
     case USG_F_LAMV: return soa_to_aos(c, s.lamv, 1, out, count, visc);
     case USG_F_GRADT:
       return s.gradTf ? pairs_to_aos(c, s.gradTf, 2, out, count, visc)
                       : pairs_to_aos(c, s.gradT, 2, out, count, visc);
-    case USG_F_TURB: return soa_to_aos(c, s.tU, 2, out, count, sst);
+    case USG_F_TURB: return pairs_to_aos(c, s.tU, 2, out, count, sst);
     case USG_F_TRES: return soa_to_aos(c, s.tR, 2, out, count, sst);
     case USG_F_MUT: return soa_to_aos(c, s.mut, 1, out, count, sst);
     case USG_F_F1: return s.F1f ? soa_to_aos(c, s.F1f, 1, out, count, sst) : soa_to_aos(c, s.F1, 1, out, count, sst);
     case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
-    case USG_F_TDW: return soa_to_aos(c, s.tdW, 2, out, count, sst);
-    case USG_F_TCONS: return soa_to_aos(c, s.tW, 2, out, count, sst);
-    case USG_F_W0: return soa_to_aos(c, s.W0, 4, out, count, true);
+    case USG_F_TDW: return pairs_to_aos(c, s.tdW, 2, out, count, sst);
+    case USG_F_TCONS: return pairs_to_aos(c, s.tW, 2, out, count, sst);
+    case USG_F_W0: return pairs_to_aos(c, s.W0, 4, out, count, true);
     case USG_F_WDIST: return soa_to_aos(c, const_cast<double *>(c->dm.wdist), 1, out, count, sst);
     case USG_F_UEDGE: {
       const int nf = c->hm.nf;
