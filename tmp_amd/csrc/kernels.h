@@ -93,7 +93,7 @@ struct DevState {
   double *lamc, *lamv;  // [nc]
   double *mu;           // [nc] laminar viscosity at the cell
   double *phi;          // [2][nc][2] (pidx(var))
-  double *R;            // [4][nc]
+  double *R;            // [2][nc][2] (pidx)
   double *dW1, *dW;     // [2][nc][2] (pidx)
   double *dt;           // [nc] LU-SGS local time step
   double *dt_rk;        // [nc] RK5 time step, frozen at stage 0
@@ -102,14 +102,14 @@ struct DevState {
                         // spectral 9, SST 10-11), oriented out of the left cell, in
                         // pairs [pair][nf][2] (rec_pair / rec_comp in the solver)
   double *D;            // [nc] LU-SGS diagonal
-  double *tD;           // [2][nc] SST diagonal
+  double *tD;           // [1][nc][2] SST diagonal (pidx)
   double *rA;           // [4][nc] LU-SGS r_A of each interior slot of a cell (from the
                         // neighbour across it), written by the face kernel
   // SST
   double *tU, *tW, *tUn, *tWn;  // [1][nc][2]: (k, omega) side by side (pidx)
   double *tgrad;                // [2][nc][2]: (d/dx, d/dy) of k, omega (pidx(2q+d))
   double *F1, *F2, *mut, *mutn; // [nc]
-  double *tR;                   // [2][nc]
+  double *tR;                   // [1][nc][2] (pidx)
   double *tdW1, *tdW;           // [1][nc][2] (pidx)
   int *err;                     // [2] invariant violation counters
   // dual time stepping: the conservative state at the two previous

@@ -722,11 +722,11 @@ __device__ __forceinline__ void fwd_store(const DevMesh &m, const DevState &s, i
   using F = Flow<FLOW>;
   const int nc = m.nc;
   double R[4], tR[2] = {0.0, 0.0}, tD[2] = {1.0, 1.0};
-  load4(s.R, nc, c, R);
+  load4p(s.R, nc, c, R);
   if (F::sst)
     for (int q = 0; q < 2; q++) {
-      tR[q] = s.tR[q * nc + c];
-      tD[q] = s.tD[q * nc + c];
+      tR[q] = s.tR[pidx(q, nc, c)];
+      tD[q] = s.tD[pidx(q, nc, c)];
     }
   fwd_store_with<FLOW>(m, s, c, R, tR, tD, dFi, tFi, diag, dW1, tdW1);
 }
@@ -783,7 +783,7 @@ __device__ __forceinline__ void bwd_store(const DevMesh &m, const DevState &s, c
   load4p(s.W, nc, c, W);
   if (F::sst) {
     for (int q = 0; q < 2; q++) {
-      tD[q] = s.tD[q * nc + c];
+      tD[q] = s.tD[pidx(q, nc, c)];
       tW[q] = s.tW[pidx(q, nc, c)];
     }
     const T *G = pickc<T>(s.grad, s.gradf);
@@ -942,14 +942,14 @@ __device__ __forceinline__ void cell_terms(const DevMesh &m, const DevState &s, 
   const unsigned char wf = m.wflag[c];
   s.lamc[c] = lamc;
   if (F::visc) s.lamv[c] = lamv;
-  store4(s.R, nc, c, Rc);
+  store4p(s.R, nc, c, Rc);
   s.dt[c] = dtc;
   s.D[c] = diag;
   if (F::sst) {
-    s.tR[c] = trc[0];
-    s.tR[nc + c] = trc[1];
-    s.tD[c] = tdiag[0];
-    s.tD[nc + c] = tdiag[1];
+    s.tR[pidx(0, nc, c)] = trc[0];
+    s.tR[pidx(1, nc, c)] = trc[1];
+    s.tD[pidx(0, nc, c)] = tdiag[0];
+    s.tD[pidx(1, nc, c)] = tdiag[1];
   }
   // cells without lower neighbours (forward wave 0): dW1 = (-R - 0) / D here,
   // and the backward update too for cells with no upper neighbour either
@@ -1150,13 +1150,13 @@ __device__ __forceinline__ void increment_of(const DevMesh &m, const DevState &s
   df_sum<FLOW, FWD>(m, s, u, dFu, tFu);
   const double diag = s.D[u];
   if (FWD) {
-    for (int j = 0; j < 4; j++) dW[j] = (-s.R[j * nc + u] - dFu[j]) / diag;
+    for (int j = 0; j < 4; j++) dW[j] = (-s.R[pidx(j, nc, u)] - dFu[j]) / diag;
     if (F::sst)
-      for (int q = 0; q < 2; q++) tdw[q] = (-s.tR[q * nc + u] - tFu[q]) / s.tD[q * nc + u];
+      for (int q = 0; q < 2; q++) tdw[q] = (-s.tR[pidx(q, nc, u)] - tFu[q]) / s.tD[pidx(q, nc, u)];
   } else {
     for (int j = 0; j < 4; j++) dW[j] = s.dW1[pidx(j, nc, u)] - dFu[j] / diag;
     if (F::sst)
-      for (int q = 0; q < 2; q++) tdw[q] = s.tdW1[pidx(q, nc, u)] - tFu[q] / s.tD[q * nc + u];
+      for (int q = 0; q < 2; q++) tdw[q] = s.tdW1[pidx(q, nc, u)] - tFu[q] / s.tD[pidx(q, nc, u)];
   }
 }
 
@@ -1290,7 +1290,7 @@ __global__ __launch_bounds__(kBlock) void k_rk_stage(const DevMesh m, const DevS
   }
   const double dtc = s.dt_rk[c];
   double Wn[4], Un[4];
-  for (int j = 0; j < 4; j++) Wn[j] = s.W0[pidx(j, nc, c)] + alpha * dtc * (-s.R[j * nc + c]) / V;
+  for (int j = 0; j < 4; j++) Wn[j] = s.W0[pidx(j, nc, c)] + alpha * dtc * (-s.R[pidx(j, nc, c)]) / V;
   prim_of(Wn, Un);
   store4p(s.W, nc, c, Wn);
   store4p(s.U, nc, c, Un);
@@ -1344,7 +1344,7 @@ __device__ void norm_partial_block(const DevMesh &m, const double *R, const int 
   const int c = b0[b] + threadIdx.x;
   double v = 0;
   if (c < b1[b] && !frozen_cell(m, c)) {
-    double r = R[c] / m.vol[c];
+    double r = R[pidx(0, m.nc, c)] / m.vol[c];
     v = r * r;
   }
   sh[threadIdx.x] = v;
@@ -2700,7 +2700,7 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
   switch (field) {
     case USG_F_PRIM: return pairs_to_aos(c, s.U, 4, out, count, true);
     case USG_F_CONS: return pairs_to_aos(c, s.W, 4, out, count, true);
-    case USG_F_RES: return soa_to_aos(c, s.R, 4, out, count, true);
+    case USG_F_RES: return pairs_to_aos(c, s.R, 4, out, count, true);
     case USG_F_GRAD:
       return s.gradf ? pairs_to_aos(c, s.gradf, 8, out, count, true) : pairs_to_aos(c, s.grad, 8, out, count, true);
     case USG_F_LAMC: return soa_to_aos(c, s.lamc, 1, out, count, true);
@@ -2713,7 +2713,7 @@ int usg_get_field(usg_ctx *c, int field, double *out, long count) {
       return s.gradTf ? pairs_to_aos(c, s.gradTf, 2, out, count, visc)
                       : pairs_to_aos(c, s.gradT, 2, out, count, visc);
     case USG_F_TURB: return pairs_to_aos(c, s.tU, 2, out, count, sst);
-    case USG_F_TRES: return soa_to_aos(c, s.tR, 2, out, count, sst);
+    case USG_F_TRES: return pairs_to_aos(c, s.tR, 2, out, count, sst);
     case USG_F_MUT: return soa_to_aos(c, s.mut, 1, out, count, sst);
     case USG_F_F1: return s.F1f ? soa_to_aos(c, s.F1f, 1, out, count, sst) : soa_to_aos(c, s.F1, 1, out, count, sst);
     case USG_F_DT: return soa_to_aos(c, s.dt, 1, out, count, true);
