@@ -162,7 +162,9 @@ __host__ __device__ __forceinline__ size_t pidx(int k, int nc, int c) {
 // The state arrays that neighbours gather -- U, W and their second buffers,
 // the sweeps' increments dW1, dW, the SST state tU, tW, tdW1, tdW, and W0,
 // Wt1, Wt2 -- use the same pairs: (rho, p), (u, v) of U; (rho, rhoE),
-// (rhou, rhov) of W; (k, omega) of tU.
+// (rhou, rhov) of W; (k, omega) of tU.  So do the residual R, tR and the SST
+// diagonal tD, which the cell kernel writes and the forward sweep reads
+// together.
 __device__ __forceinline__ void load4p(const double *a, int nc, int c, double *v) {
   const double2 x = ld2(a, 0, nc, c), y = ld2(a, 1, nc, c);
   v[0] = x.x;
