@@ -130,7 +130,9 @@ int usg_sync(usg_ctx *ctx);
  * LUSGSIntegrator::initializeData (LUSGS_Integrator.cpp:107-111); SST adds
  * SSTkomegaModel::InitializeturbulenceVariable (SSTKWStrategy.h:582-613). */
 int usg_init_freestream(usg_ctx *ctx);
-/* Warm start from primitive U [nc*4] (and, for SST, (k,omega) [nc*2], may be NULL). */
+/* Warm start from primitive U [nc*4] (and, for SST, (k,omega) [nc*2], may be
+ * NULL: (k, omega) stay as they are).  For SST the conservative (rho k,
+ * rho omega) is recomputed from (k, omega) and the new density either way. */
 int usg_set_prim(usg_ctx *ctx, const double *U, const double *turb);
 /* Restart: the whole pseudo-time state, as usg_get_field reads it --
  * U [nc*4] (USG_F_PRIM), W [nc*4] (USG_F_CONS; NULL: computed from U), and

@@ -104,6 +104,26 @@ def test_lusgs_trajectory(name, mk, flow, mach, aoa, re, n):
         assert et <= TOL_TRAJ
 
 
+@pytest.mark.parametrize("name,mk,flow,mach,aoa,re", [CASES[1], CASES[3]], ids=["bump", "airfoil-sst"])
+def test_sweep_increments_and_state_fields(name, mk, flow, mach, aoa, re):
+    """One LU-SGS iteration from a perturbed state: every component of the
+    forward and backward increments dW1, dW (and the SST tdW1, tdW), the
+    conservative state W and the SST (rho k, rho omega) against the oracle's,
+    through the field exports (LUSGS_Integrator.cpp:492-561)."""
+    m = mk()
+    orc, gpu, om = _pair(m, flow, mach, aoa, re)
+    orc.lusgs_step()
+    gpu.lusgs(1)
+    checks = {"dW1": (usg.F_DW1, "dW1", 4), "dW": (usg.F_DW, "dW", 4), "W": (usg.F_CONS, "W", 4)}
+    if flow == O.SST:
+        checks.update({"tdW": (usg.F_TDW, "tdW", 2), "tW": (usg.F_TCONS, "tW", 2)})
+    for k, (fid, on, w) in checks.items():
+        g, c = gpu.field(fid), orc.field(on, w)
+        e = np.max(relerr(g, c))
+        print(f"{name} {k}: rel {e:.3e}")
+        assert np.all(np.isfinite(g)) and e <= TOL_TRAJ, (k, e)
+
+
 @pytest.mark.parametrize("name,mk,flow", [("part2-euler", part2, O.EULER),
                                           ("airfoil-laminar", airfoil_small, O.LAMINAR)],
                          ids=["part2-euler", "airfoil-laminar"])

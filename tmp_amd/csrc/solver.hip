@@ -2456,11 +2456,15 @@ int usg_set_prim(usg_ctx *c, const double *U, const double *turb) {
     for (int j = 0; j < 4; j++) soa[pidx(j, nc, i)] = U[4 * (size_t)i + j];
   HIPCHK(hipMemcpyAsync(c->ds.U, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_cons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds, c->ds.W);
-  if (c->prm.flow == USG_SST && turb) {
-    std::vector<double> t(2 * (size_t)nc);
-    for (int i = 0; i < nc; i++)
-      for (int j = 0; j < 2; j++) t[pidx(j, nc, i)] = turb[2 * (size_t)i + j];
-    HIPCHK(hipMemcpyAsync(c->ds.tU, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (c->prm.flow == USG_SST) {
+    // (k, omega) as given, or as they are; (rho k, rho omega) from them and
+    // the new density either way (SSTkomegaModel::initializeConsData)
+    std::vector<double> t(turb ? 2 * (size_t)nc : 0);
+    if (turb) {
+      for (int i = 0; i < nc; i++)
+        for (int j = 0; j < 2; j++) t[pidx(j, nc, i)] = turb[2 * (size_t)i + j];
+      HIPCHK(hipMemcpyAsync(c->ds.tU, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
     hipLaunchKernelGGL(k_tcons, dim3(c->nblk), dim3(kBlock), 0, c->stream, c->dm, c->ds);
   }
   HIPCHK(hipGetLastError());
