@@ -291,13 +291,11 @@ __device__ __forceinline__ T sutherland(const Consts &k, T Tc) {
 // Venkatakrishnan limiter value for one face (Vankatakrishnan_Limiter.hpp:128-139)
 template <typename T>
 __device__ __forceinline__ T venkat(T d1max, T d1min, T delta2, T eps2) {
-  if (delta2 > 0)
-    return (d1max * d1max + eps2 + 2 * delta2 * d1max) /
-           (d1max * d1max + 2 * delta2 * delta2 + d1max * delta2 + eps2);
-  if (delta2 < 0)
-    return (d1min * d1min + eps2 + 2 * delta2 * d1min) /
-           (d1min * d1min + 2 * delta2 * delta2 + d1min * delta2 + eps2);
-  return 1;
+  // one quotient, with the bound on the side delta2 points to (lanes that
+  // take different sides do not run both divisions)
+  const T d = delta2 > 0 ? d1max : d1min;
+  const T r = (d * d + eps2 + 2 * delta2 * d) / (d * d + 2 * delta2 * delta2 + d * delta2 + eps2);
+  return (delta2 > 0 || delta2 < 0) ? r : T(1);
 }
 
 // The p* invariant of the interior HLLC flux (HLLCFluxStrategy.cpp:77-86):
