@@ -326,7 +326,9 @@ __device__ __forceinline__ bool hllc(const T *UL, const T *UR, T n0, T n1, T are
   T efficient_righ = 1 - efficient_left;
   T enthalpy_tilda = efficient_left * enthalpy_left + efficient_righ * enthalpy_righ;
   T Vn_tilda = efficient_left * Vn_left + efficient_righ * Vn_righ;
-  T soundspeed_tilda = sqrt((Gamma - 1) * (enthalpy_tilda - T(0.5) * pow(Vn_tilda, T(2.0))));
+  // the reference's pow(Vn, 2) as the one rounded product: the correctly
+  // rounded square, where a general pow costs ~200 instructions here
+  T soundspeed_tilda = sqrt((Gamma - 1) * (enthalpy_tilda - T(0.5) * (Vn_tilda * Vn_tilda)));
   T Sl = smin(Vn_left - soundspeed_left, Vn_tilda - soundspeed_tilda);
   T Sr = smax(Vn_tilda + soundspeed_tilda, Vn_righ + soundspeed_righ);
   T Sm = (density_righ * Vn_righ * (Sr - Vn_righ) - density_left * Vn_left * (Sl - Vn_left) + pressure_left -
