@@ -80,7 +80,7 @@ def time_to_tolerance(n_train, n_test, epochs):
     first residual on held-out laminar NACA 4-digit cases, for the plain
     cold start (each case alone, from the free stream), the batched cold
     start, and the batched NN warm start (inference and the cold-start
-    judge included), with the network trained in this run on fields this
+    screen included), with the network trained in this run on fields this
     solver converged (tmp_amd/warmstart.py)."""
     from tmp_amd import warmstart
     r = warmstart.experiment(n_train=n_train, n_test=n_test, epochs=epochs)
@@ -90,7 +90,7 @@ def time_to_tolerance(n_train, n_test, epochs):
         "family": r["family"], "train_cases": r["train_cases"], "test_cases": r["test_cases"],
         "epochs": r["epochs"], "train_seconds": r["train_seconds"], "tolerance": "1e-8 x cold first residual",
         "seconds_cold_plain": t["cold_plain_single_case_solves"], "seconds_cold_batched": t["cold_batched"],
-        "seconds_warm_batched": t["warm_batched_incl_inference_and_judge"],
+        "seconds_warm_batched": t["warm_batched_incl_inference_and_screen"],
         "speedup_warm_batched_vs_cold_plain": r["measured_speedup_warm_batched_vs_cold_plain"],
         "speedup_warm_batched_vs_cold_batched": r["measured_speedup_warm_batched_vs_cold_batched"],
         "median_iteration_speedup": lv["median_iteration_speedup"],

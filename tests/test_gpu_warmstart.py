@@ -1,4 +1,4 @@
-"""The learned warm start's fallback and judge on the GPU: predictions
+"""The learned warm start's fallback and screen on the GPU: predictions
 corrupted on the host -- a NaN, a negative pressure, a finite field far
 from the solution -- are refused, and those cases then follow the cold
 start (the free stream) bit for bit, while a sound prediction is kept."""
@@ -10,7 +10,7 @@ from tmp_amd import usg, warmstart
 pytestmark = pytest.mark.gpu
 
 
-def test_judge_keeps_sound_and_reverts_corrupted_predictions():
+def test_screen_keeps_sound_and_reverts_corrupted_predictions():
     cases = warmstart.family_cases(4, 11)
     ms = [warmstart.family_mesh(c) for c in cases]
     ref = warmstart.batch_solver(ms, cases)
@@ -21,7 +21,7 @@ def test_judge_keeps_sound_and_reverts_corrupted_predictions():
     U[cell[2] + 11, 1] = -0.3                      # case 2: negative pressure
     U[cell[3]:cell[4], 2:] *= -3.0                 # case 3: finite, reversed and tripled velocity
     s = warmstart.batch_solver(ms, cases)
-    kept, r_cold, nprobe = warmstart.judged_start(s, U)
+    kept, r_cold, nprobe = warmstart.screened_start(s, U)
     print(f"kept {kept.tolist()} after {nprobe} probe iterations; cold first residuals {r_cold}", flush=True)
     assert kept.tolist() == [True, False, False, False]
     assert np.all(s.status() == 0)
@@ -36,15 +36,15 @@ def test_judge_keeps_sound_and_reverts_corrupted_predictions():
     assert s.res_l2()[0] < 0.1 * c.res_l2()[0]
 
 
-def test_judge_on_a_single_case_context():
+def test_screen_on_a_single_case_context():
     """A single-case context fails the probe call when a check breaks; the
-    judge then restarts it from the free stream."""
+    screen then restarts it from the free stream."""
     case = warmstart.family_cases(1, 12)[0]
     m = warmstart.family_mesh(case)
     s = usg.Solver(m, flow=usg.LAMINAR, mach=case["mach"], aoa=case["aoa"], reynolds=case["re"])
     U = np.tile(s.freestream(), (m.nc, 1))
     U[:, 2:] *= -3.0
-    kept, r_cold, _ = warmstart.judged_start(s, U)
+    kept, r_cold, _ = warmstart.screened_start(s, U)
     assert kept.tolist() == [False]
     assert np.array_equal(s.U, np.tile(s.freestream(), (m.nc, 1)))
 

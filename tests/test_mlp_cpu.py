@@ -76,7 +76,7 @@ def test_mlp_exports_and_shape_checks():
 
 
 def test_warm_start_screen():
-    """The judge's screen of a predicted field (tmp_amd.warmstart):
+    """The screen of a predicted field (tmp_amd.warmstart):
     finite, positive density and pressure everywhere."""
     from tmp_amd import warmstart
     U = np.tile([1.0, 1.0, 0.3, 0.0], (10, 1))

@@ -208,12 +208,12 @@ def screen_prediction(U: np.ndarray) -> bool:
     return bool(np.all(np.isfinite(U)) and np.all(U[:, 0] > 0) and np.all(U[:, 1] > 0))
 
 
-def judged_start(solver, U_pred: np.ndarray, probe: int = 20, growth: float = 10.0):
-    """Warm start with the plain cold start as fallback and judge, per case.
+def screened_start(solver, U_pred: np.ndarray, probe: int = 20, growth: float = 10.0):
+    """Warm start with the plain cold start as fallback and screen, per case.
 
     The cold start (the free stream, Euler2D.cpp:153-171) fixes the bar: its
     first residual r_cold.  A case starts from its prediction if the
-    prediction is finite with positive density and pressure, and is judged
+    prediction is finite with positive density and pressure, and is screened
     after ``probe`` LU-SGS iterations: it keeps the warm start if every
     residual of the probe is finite, the case passed the solver's checks,
     and its last residual is below both r_cold and ``growth`` times its own
@@ -310,13 +310,13 @@ def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=
     cl_c, cd_c = cold.forces()
     # plain: each case alone in its own context from the free stream
     t_plain = plain_solves(ms, test_cases, tol, max_iters)
-    # warm: network prediction, judged against the cold start
+    # warm: network prediction, screened against the cold start
     warm = batch_solver(ms, test_cases)
     warm.sync()
     t0 = time.perf_counter()
     U0 = np.concatenate([fields(net.predict(features(m, c)), c) for m, c in zip(ms, test_cases)])
     t_pred = time.perf_counter() - t0
-    warm_kept, _, nprobe = judged_start(warm, U0)
+    warm_kept, _, nprobe = screened_start(warm, U0)
     n_warm, h_warm = iterate_to(warm, tol, max_iters, log=log)
     t_warm_batch = time.perf_counter() - t0
     n_warm = np.where(n_warm > 0, n_warm + nprobe, 0)
@@ -354,7 +354,7 @@ def experiment(n_train=256, n_test=16, train_iters=20000, epochs=150, max_iters=
         "measured_seconds_to_1e-8": {
             "cold_plain_single_case_solves": t_plain,
             "cold_batched": t_cold_batch,
-            "warm_batched_incl_inference_and_judge": t_warm_batch},
+            "warm_batched_incl_inference_and_screen": t_warm_batch},
         "measured_speedup_warm_batched_vs_cold_batched": t_cold_batch / t_warm_batch,
         "measured_speedup_warm_batched_vs_cold_plain": t_plain / t_warm_batch,
         "total_seconds": time.perf_counter() - t_all,
